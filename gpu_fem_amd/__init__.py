@@ -1,0 +1,386 @@
+"""gpu_fem_amd: MI355X (gfx950) linear-static MITC4 shell path.
+
+The product is the C-ABI shared library ``libgpufem_amd.so`` (declared in
+``include/gpufem_amd.h``) and the C++ host classes in ``include/gpufem/``.
+This module loads the library with ctypes for Python callers (tests, the
+benchmark); device buffers are passed as raw pointers (torch tensors are
+used only as device allocations).  There is no CPU fallback: every device
+entry point requires the HIP library, and loading fails loudly when it is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import meshes  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpufem_amd.so")
+_LIB = None
+
+GF_OK = 0
+STATUS = {0: "GF_OK", -1: "GF_ERR_INVALID", -2: "GF_ERR_NOMEM",
+          -3: "GF_ERR_DEVICE", -4: "GF_ERR_ZERO_PIVOT", -5: "GF_ERR_BREAKDOWN",
+          -6: "GF_ERR_NO_CONVERGENCE", -7: "GF_ERR_OVER_BUDGET",
+          -8: "GF_ERR_TIMEOUT"}
+
+
+class GfError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("reorder", ctypes.c_int), ("lev_fill", ctypes.c_int),
+                ("restart", ctypes.c_int), ("max_iters", ctypes.c_int),
+                ("rtol", ctypes.c_double), ("precision", ctypes.c_int)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("outer", ctypes.c_int), ("rel_resid", ctypes.c_double),
+                ("t_assemble", ctypes.c_double), ("t_factor", ctypes.c_double),
+                ("t_solve", ctypes.c_double)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_PI = ctypes.POINTER(ctypes.c_int)
+_PPI = ctypes.POINTER(_PI)
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+_SIGS = {
+    "gf_version": [],
+    "gf_context_create": [_I, ctypes.POINTER(_P)],
+    "gf_context_destroy": [_P],
+    "gf_context_synchronize": [_P],
+    "gf_context_last_error": [_P],
+    "gf_context_stream": [_P],
+    "gf_free": [_P],
+    "gf_node_graph": [_I, _I, _I, _P, _PPI, _PPI, _PI],
+    "gf_rcm_order": [_I, _P, _P, _I, _I, _P],
+    "gf_amd_order": [_I, _P, _P, _P],
+    "gf_pattern_permute": [_I, _P, _P, _P, _PPI, _PPI],
+    "gf_iluk_symbolic": [_I, _P, _P, _I, _PPI, _PPI, _PI],
+    "gf_elem_ind_map": [_I, _I, _P, _P, _P, _P, _P],
+    "gf_transpose_map": [_I, _P, _P, _PPI, _PPI, _PPI],
+    "gf_bandwidth": [_I, _P, _P],
+    "gf_assemble_jacobian_mitc4_f64": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P,
+                                       _P],
+    "gf_assemble_residual_mitc4_f64": [_P, _I, _I, _P, _P, _P, _P, _P, _P],
+    "gf_energy_mitc4_f64": [_P, _I, _P, _P, _P, _P, _P],
+    "gf_bc_apply_mat": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    "gf_bc_apply_vec": [_P, _I, _P, _P, _P],
+    "gf_bsr_copy_to_pattern": [_P, _I, _P, _P, _P, _P, _P, _P],
+    "gf_permute_vec": [_P, _I, _P, _P, _P],
+    "gf_bsr_spmv_f64": [_P, _I, _P, _P, _P, _P, _P],
+    "gf_bsr_spmv_f32": [_P, _I, _P, _P, _P, _P, _P],
+    "gf_convert_f64_f32": [_P, ctypes.c_size_t, _P, _P],
+    "gf_ilu_create": [_P, _I, _P, _P, ctypes.POINTER(_P)],
+    "gf_ilu_destroy": [_P],
+    "gf_ilu_info": [_P, _PI, _PI, _PI],
+    "gf_ilu_device_rowp": [_P],
+    "gf_ilu_device_cols": [_P],
+    "gf_bsr_ilu_factor_f64": [_P, _P, _PI],
+    "gf_bsr_ilu_factor_f32": [_P, _P, _PI],
+    "gf_bsr_ilu_apply_f64": [_P, _P, _P, _P, _P],
+    "gf_bsr_ilu_apply_f32": [_P, _P, _P, _P, _P],
+    "gf_options_default": [ctypes.POINTER(Options)],
+    "gf_problem_create": [_P, _I, _I, _P, _P, _P, _I, _P,
+                          ctypes.POINTER(Options), ctypes.POINTER(_P)],
+    "gf_problem_destroy": [_P],
+    "gf_problem_assemble": [_P],
+    "gf_problem_factor": [_P],
+    "gf_problem_solve": [_P, _P, _P, _I, ctypes.POINTER(SolveInfo)],
+    "gf_problem_sizes": [_P, _PI, _PI, _PI],
+    "gf_problem_device_arrays": [_P] + [ctypes.POINTER(_P)] * 7,
+    "gf_problem_host_perm": [_P, _P],
+}
+_RESTYPES = {"gf_context_last_error": ctypes.c_char_p,
+             "gf_context_stream": _P, "gf_free": None,
+             "gf_options_default": None, "gf_ilu_device_rowp": _P,
+             "gf_ilu_device_cols": _P}
+
+
+def exported_symbols():
+    """Names declared in include/gpufem_amd.h (parsed from the header)."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "gpufem_amd.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(gf_[a-z0-9_]+)\s*\(", txt)))
+
+
+def build(force=False):
+    from .build import build as _b
+    return _b(force=force)
+
+
+def lib():
+    """Load libgpufem_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: run __graft_entry__.build() or "
+                "python gpu_fem_amd/build.py (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, ctypes.c_int)
+        _LIB = L
+    return _LIB
+
+
+def check(code, ctx=None):
+    if code != GF_OK:
+        msg = ""
+        if ctx is not None:
+            msg = lib().gf_context_last_error(ctx).decode()
+        raise GfError(code, msg)
+    return code
+
+
+def ptr(a):
+    """Raw pointer of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(_P)
+    return _P(a.data_ptr())
+
+
+def _take_host(p, n):
+    out = np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].copy()
+    lib().gf_free(ctypes.cast(p, _P))
+    return out.astype(np.int32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# --------------------------------------------------------- host symbolic
+def node_graph(conn, nnodes, nodes_per_elem=4):
+    conn = _i32(conn).ravel()
+    rp, cp, nnz = _PI(), _PI(), ctypes.c_int()
+    check(lib().gf_node_graph(len(conn) // nodes_per_elem, nnodes,
+                              nodes_per_elem, ptr(conn), ctypes.byref(rp),
+                              ctypes.byref(cp), ctypes.byref(nnz)))
+    return _take_host(rp, nnodes + 1), _take_host(cp, nnz.value)
+
+
+def rcm_order(rowp, cols, root=0, iters=1):
+    rowp, cols = _i32(rowp), _i32(cols)
+    out = np.zeros(len(rowp) - 1, np.int32)
+    check(lib().gf_rcm_order(len(rowp) - 1, ptr(rowp), ptr(cols), root, iters,
+                             ptr(out)))
+    return out
+
+
+def amd_order(rowp, cols):
+    rowp, cols = _i32(rowp), _i32(cols)
+    out = np.zeros(len(rowp) - 1, np.int32)
+    check(lib().gf_amd_order(len(rowp) - 1, ptr(rowp), ptr(cols), ptr(out)))
+    return out
+
+
+def pattern_permute(rowp, cols, perm):
+    rowp, cols, perm = _i32(rowp), _i32(cols), _i32(perm)
+    n = len(rowp) - 1
+    rp, cp = _PI(), _PI()
+    check(lib().gf_pattern_permute(n, ptr(rowp), ptr(cols), ptr(perm),
+                                   ctypes.byref(rp), ctypes.byref(cp)))
+    return _take_host(rp, n + 1), _take_host(cp, int(rowp[-1]))
+
+
+def iluk_symbolic(rowp, cols, lev_fill):
+    rowp, cols = _i32(rowp), _i32(cols)
+    n = len(rowp) - 1
+    rp, cp, nnz = _PI(), _PI(), ctypes.c_int()
+    check(lib().gf_iluk_symbolic(n, ptr(rowp), ptr(cols), lev_fill,
+                                 ctypes.byref(rp), ctypes.byref(cp),
+                                 ctypes.byref(nnz)))
+    return _take_host(rp, n + 1), _take_host(cp, nnz.value)
+
+
+def elem_ind_map(conn, rowp, cols, perm, nodes_per_elem=4):
+    conn = _i32(conn).ravel()
+    ne = len(conn) // nodes_per_elem
+    out = np.zeros(ne * nodes_per_elem ** 2, np.int32)
+    rowp, cols, perm = _i32(rowp), _i32(cols), _i32(perm)
+    check(lib().gf_elem_ind_map(ne, nodes_per_elem, ptr(conn), ptr(rowp),
+                                ptr(cols), ptr(perm), ptr(out)))
+    return out
+
+
+def transpose_map(rowp, cols):
+    rowp, cols = _i32(rowp), _i32(cols)
+    n = len(rowp) - 1
+    a, b, c = _PI(), _PI(), _PI()
+    check(lib().gf_transpose_map(n, ptr(rowp), ptr(cols), ctypes.byref(a),
+                                 ctypes.byref(b), ctypes.byref(c)))
+    nnz = int(rowp[-1])
+    return _take_host(a, n + 1), _take_host(b, nnz), _take_host(c, nnz)
+
+
+def bandwidth(rowp, cols):
+    rowp, cols = _i32(rowp), _i32(cols)
+    return lib().gf_bandwidth(len(rowp) - 1, ptr(rowp), ptr(cols))
+
+
+# --------------------------------------------------------- device objects
+class Context:
+    """A device + HIP stream (gf_context)."""
+
+    def __init__(self, device=0):
+        h = _P()
+        check(lib().gf_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def check(self, code):
+        return check(code, self.h)
+
+    def sync(self):
+        self.check(lib().gf_context_synchronize(self.h))
+
+    @property
+    def stream(self):
+        return lib().gf_context_stream(self.h)
+
+    def close(self):
+        if self.h:
+            lib().gf_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ilu:
+    """ILU analysis on a block pattern (gf_ilu)."""
+
+    def __init__(self, ctx: Context, rowp, cols):
+        self.ctx = ctx
+        self.rowp, self.cols = _i32(rowp), _i32(cols)
+        h = _P()
+        ctx.check(lib().gf_ilu_create(ctx.h, len(rowp) - 1, ptr(self.rowp),
+                                      ptr(self.cols), ctypes.byref(h)))
+        self.h = h
+
+    def info(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().gf_ilu_info(self.h, ctypes.byref(a), ctypes.byref(b),
+                          ctypes.byref(c))
+        return {"nnzb": a.value, "levels_lower": b.value,
+                "levels_upper": c.value}
+
+    def factor(self, d_vals):
+        zp = ctypes.c_int(-1)
+        fn = (lib().gf_bsr_ilu_factor_f64 if d_vals.element_size() == 8
+              else lib().gf_bsr_ilu_factor_f32)
+        code = fn(self.h, ptr(d_vals), ctypes.byref(zp))
+        if code != GF_OK:
+            raise GfError(code, lib().gf_context_last_error(self.ctx.h).decode())
+        return zp.value
+
+    def apply(self, d_lu, d_b, d_tmp, d_x):
+        fn = (lib().gf_bsr_ilu_apply_f64 if d_lu.element_size() == 8
+              else lib().gf_bsr_ilu_apply_f32)
+        self.ctx.check(fn(self.h, ptr(d_lu), ptr(d_b), ptr(d_tmp), ptr(d_x)))
+
+    def close(self):
+        if self.h:
+            lib().gf_ilu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().gf_options_default(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Problem:
+    """A complete linear-static problem on the device (gf_problem)."""
+
+    def __init__(self, ctx: Context, mesh, **opts):
+        self.ctx = ctx
+        self.mesh = mesh
+        self.opt = default_options(**opts)
+        conn = _i32(mesh.conn).ravel()
+        xpts = np.ascontiguousarray(mesh.xpts, np.float64).ravel()
+        data = np.ascontiguousarray(mesh.data, np.float64).ravel()
+        bcs = _i32(mesh.bcs)
+        h = _P()
+        ctx.check(lib().gf_problem_create(
+            ctx.h, mesh.nnodes, mesh.nelems, ptr(conn), ptr(xpts), ptr(data),
+            len(bcs), ptr(bcs), ctypes.byref(self.opt), ctypes.byref(h)))
+        self.h = h
+
+    def sizes(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(lib().gf_problem_sizes(self.h, ctypes.byref(a),
+                                              ctypes.byref(b),
+                                              ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def device_arrays(self):
+        ps = [_P() for _ in range(7)]
+        self.ctx.check(lib().gf_problem_device_arrays(
+            self.h, *[ctypes.byref(p) for p in ps]))
+        return dict(zip(["krowp", "kcols", "kvals", "mrowp", "mcols", "mvals",
+                         "perm"], [p.value for p in ps]))
+
+    def perm(self):
+        out = np.zeros(self.mesh.nnodes, np.int32)
+        self.ctx.check(lib().gf_problem_host_perm(self.h, ptr(out)))
+        return out
+
+    def assemble(self):
+        self.ctx.check(lib().gf_problem_assemble(self.h))
+
+    def factor(self):
+        self.ctx.check(lib().gf_problem_factor(self.h))
+
+    def solve(self, loads=None, x=None, raise_on_error=True):
+        """Solve with host (numpy) or device (torch) loads; returns (x, info)."""
+        loads = self.mesh.loads if loads is None else loads
+        on_dev = not isinstance(loads, np.ndarray)
+        if x is None:
+            x = (np.zeros(6 * self.mesh.nnodes) if not on_dev
+                 else loads.new_zeros(loads.shape))
+        if not on_dev:
+            loads = np.ascontiguousarray(loads, np.float64)
+        info = SolveInfo()
+        code = lib().gf_problem_solve(self.h, ptr(loads), ptr(x), int(on_dev),
+                                      ctypes.byref(info))
+        if code != GF_OK and raise_on_error:
+            raise GfError(code, lib().gf_context_last_error(self.ctx.h).decode())
+        return x, info
+
+    def close(self):
+        if self.h:
+            lib().gf_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

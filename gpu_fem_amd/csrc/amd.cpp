@@ -1,0 +1,116 @@
+// Approximate minimum degree ordering on the node graph.
+//
+// The reference orders with sparse-utils' BSRMatAMDFactorSymbolicCUDA
+// (include/linalg/bsr_utils.h:510-567), whose source is not part of the
+// reference tree.  This is an independent implementation of the published
+// quotient-graph method (Amestoy, Davis & Duff, SIAM J. Matrix Anal. Appl.
+// 17(4), 1996): eliminated nodes become elements, adjacent elements are
+// absorbed into the new one, and each variable's degree is bounded by
+//   d_i ~ |A_i| + |L_p \ i| + sum_{e in E_i, e != p} |L_e \ L_p|
+// (|L_e \ L_p| via the w(e) counters).  Supervariable detection and
+// aggressive absorption are omitted; ties go to the most recently updated
+// node (degree buckets are stacks).  Output perm[old] = new.
+#include <algorithm>
+#include <vector>
+
+#include "symbolic.h"
+
+namespace gf {
+
+int amd_order(int n, const int* rowp, const int* cols, int* perm) {
+  if (n <= 0) return GF_ERR_INVALID;
+  std::vector<std::vector<int>> vars(n), elems(n), le(n);
+  for (int i = 0; i < n; i++)
+    for (int k = rowp[i]; k < rowp[i + 1]; k++)
+      if (cols[k] != i) vars[i].push_back(cols[k]);
+  std::vector<char> eliminated(n, 0), dead(n, 0);
+  std::vector<int> deg(n), head(n + 1, -1), nxt(n, -1), prv(n, -1);
+  std::vector<int> mark(n, -1), w(n, -1);
+  auto bucket_insert = [&](int i) {
+    const int d = deg[i];
+    nxt[i] = head[d];
+    prv[i] = -1;
+    if (head[d] >= 0) prv[head[d]] = i;
+    head[d] = i;
+  };
+  auto bucket_remove = [&](int i) {
+    if (prv[i] >= 0)
+      nxt[prv[i]] = nxt[i];
+    else
+      head[deg[i]] = nxt[i];
+    if (nxt[i] >= 0) prv[nxt[i]] = prv[i];
+  };
+  for (int i = n - 1; i >= 0; i--) {
+    deg[i] = std::min((int)vars[i].size(), n);
+    bucket_insert(i);
+  }
+  int mindeg = 0;
+  std::vector<int> Lp;
+  for (int k = 0; k < n; k++) {
+    while (mindeg <= n && head[mindeg] < 0) mindeg++;
+    const int p = head[mindeg];
+    bucket_remove(p);
+    eliminated[p] = 1;
+    perm[p] = k;
+    // L_p = (A_p u (union of L_e, e in E_p)) \ {p}
+    Lp.clear();
+    mark[p] = k;
+    for (int v : vars[p])
+      if (!eliminated[v] && mark[v] != k) {
+        mark[v] = k;
+        Lp.push_back(v);
+      }
+    for (int e : elems[p]) {
+      if (dead[e]) continue;
+      for (int v : le[e])
+        if (!eliminated[v] && mark[v] != k) {
+          mark[v] = k;
+          Lp.push_back(v);
+        }
+      dead[e] = 1;  // absorbed into p
+      le[e].clear();
+      le[e].shrink_to_fit();
+    }
+    vars[p].clear();
+    vars[p].shrink_to_fit();
+    elems[p].clear();
+    le[p] = Lp;
+    // w(e) = |L_e \ L_p| for elements adjacent to L_p
+    for (int i : Lp)
+      for (int e : elems[i]) {
+        if (dead[e]) continue;
+        if (w[e] < 0) w[e] = (int)le[e].size();
+        w[e]--;
+      }
+    const int lp = (int)Lp.size();
+    for (int i : Lp) {
+      bucket_remove(i);
+      // prune: eliminated/L_p variables leave A_i; dead elements leave E_i
+      auto& A = vars[i];
+      A.erase(std::remove_if(A.begin(), A.end(),
+                             [&](int v) { return eliminated[v] || mark[v] == k; }),
+              A.end());
+      auto& E = elems[i];
+      E.erase(std::remove_if(E.begin(), E.end(), [&](int e) { return dead[e]; }),
+              E.end());
+      long d = (long)A.size() + (lp - 1);
+      for (int e : E) d += std::max(w[e], 0);
+      E.push_back(p);
+      const long bound = std::min<long>((long)deg[i] + lp, (long)(n - k - 1));
+      deg[i] = (int)std::max<long>(0, std::min(d, bound));
+      bucket_insert(i);
+      mindeg = std::min(mindeg, deg[i]);
+    }
+    for (int i : Lp)
+      for (int e : elems[i]) w[e] = -1;
+  }
+  return GF_OK;
+}
+
+}  // namespace gf
+
+extern "C" int gf_amd_order(int n, const int* rowp, const int* cols,
+                            int* perm) {
+  if (n <= 0 || !rowp || !cols || !perm) return GF_ERR_INVALID;
+  return gf::amd_order(n, rowp, cols, perm);
+}

@@ -1,0 +1,268 @@
+// MITC4 Jacobian / residual / energy assembly kernels (gfx950).
+//
+// Replaces add_jacobian_gpu (include/base/elem_group.cuh:101-211), the
+// residual path add_residual_cpu (include/base/elem_group.h:40-79) and
+// add_energy (assembler.h:198-229).
+//
+// Layout: one 64-lane workgroup per element.
+//   stage 1  lanes 0-3: node frames; lanes 4-8: tying-point geometry
+//   stage 2  lanes 0-3: quadrature frames; lanes 4-8: tying normals
+//   stage 3  lanes 0-23: column j of B_q (4 x 9) and D_q = detXd C B_q -> LDS
+//   stage 4  all 64 lanes: a 3x3 tile of K_e = sum_q B_q^T D_q (8x8 tiles)
+//   stage 5  scatter through elemIndMap (global FP64 atomics), residual
+//            R_e = K_e u_e reduced through LDS.
+#include "common.h"
+#include "mitc4.h"
+
+namespace gf {
+namespace {
+
+struct ElemLDS {
+  double X[12];
+  double u[24];
+  double data[4];
+  mitc4::Geom g;
+  double B[4][9][24];
+  double D[4][9][24];
+  double rpart[8][24];  // residual partial sums per tile column J
+};
+
+__device__ __forceinline__ void load_element(ElemLDS& s, int e, int lane,
+                                             const int* conn,
+                                             const double* xpts,
+                                             const double* vars,
+                                             const double* data) {
+  if (lane < 12) {
+    const int a = lane / 3, k = lane % 3;
+    s.X[lane] = xpts[3 * (size_t)conn[4 * (size_t)e + a] + k];
+  } else if (lane < 36) {
+    const int i = lane - 12, a = i / 6, k = i % 6;
+    s.u[i] = vars ? vars[6 * (size_t)conn[4 * (size_t)e + a] + k] : 0.0;
+  } else if (lane < 40) {
+    s.data[lane - 36] = data[4 * (size_t)e + (lane - 36)];
+  }
+}
+
+__device__ __forceinline__ void element_geometry(ElemLDS& s, int lane) {
+  if (lane < 4) {
+    mitc4::geom_node(lane, s.X, s.g);
+  } else if (lane < 9) {
+    mitc4::geom_tying_x(lane - 4, s.X, s.g);
+  }
+  __syncthreads();
+  if (lane < 4) {
+    mitc4::geom_quad(lane, s.X, s.g);
+  } else if (lane < 9) {
+    mitc4::geom_tying_n(lane - 4, s.g);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void element_bd(ElemLDS& s, int lane) {
+  if (lane < 24) {
+    double col[4][9];
+    mitc4::b_column(lane, s.g, col);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      double st[9];
+      mitc4::stress(s.data, col[q], st);
+      const double sc = s.g.scale[q];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        s.B[q][k][lane] = col[q][k];
+        s.D[q][k][lane] = sc * st[k];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// 3x3 tile (rows 3I.., cols 3J..) of K_e
+__device__ __forceinline__ void element_tile(const ElemLDS& s, int I, int J,
+                                             double acc[3][3]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[r][c] = 0.0;
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double b0 = s.B[q][k][3 * I], b1 = s.B[q][k][3 * I + 1],
+                   b2 = s.B[q][k][3 * I + 2];
+      const double d0 = s.D[q][k][3 * J], d1 = s.D[q][k][3 * J + 1],
+                   d2 = s.D[q][k][3 * J + 2];
+      acc[0][0] += b0 * d0;
+      acc[0][1] += b0 * d1;
+      acc[0][2] += b0 * d2;
+      acc[1][0] += b1 * d0;
+      acc[1][1] += b1 * d1;
+      acc[1][2] += b1 * d2;
+      acc[2][0] += b2 * d0;
+      acc[2][1] += b2 * d1;
+      acc[2][2] += b2 * d2;
+    }
+}
+
+__global__ void __launch_bounds__(64)
+    jacobian_atomic_kernel(int nelems, const int* __restrict__ conn,
+                           const double* __restrict__ xpts,
+                           const double* __restrict__ vars,
+                           const double* __restrict__ data,
+                           const int* __restrict__ elem_ind_map,
+                           const int* __restrict__ perm, double* kvals,
+                           double* res) {
+  __shared__ ElemLDS s;
+  const int e = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (e >= nelems) return;
+  load_element(s, e, lane, conn, xpts, vars, data);
+  __syncthreads();
+  element_geometry(s, lane);
+  element_bd(s, lane);
+  const int I = lane / 8, J = lane % 8;
+  double acc[3][3];
+  element_tile(s, I, J, acc);
+  // scatter: tile rows belong to node a = I/2, local rows 3*(I%2)..
+  const int a = I / 2, b = J / 2, r0 = 3 * (I % 2), c0 = 3 * (J % 2);
+  const size_t blk = (size_t)elem_ind_map[16 * (size_t)e + 4 * a + b];
+  double* dst = kvals + 36 * blk;
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      unsafeAtomicAdd(&dst[6 * (r0 + r) + c0 + c], acc[r][c]);
+  if (res) {
+    double p0 = 0, p1 = 0, p2 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double uc = s.u[3 * J + c];
+      p0 += acc[0][c] * uc;
+      p1 += acc[1][c] * uc;
+      p2 += acc[2][c] * uc;
+    }
+    s.rpart[J][3 * I] = p0;
+    s.rpart[J][3 * I + 1] = p1;
+    s.rpart[J][3 * I + 2] = p2;
+    __syncthreads();
+    if (lane < 24) {
+      double r = 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; j++) r += s.rpart[j][lane];
+      const int node = conn[4 * (size_t)e + lane / 6];
+      const int row = perm ? perm[node] : node;
+      unsafeAtomicAdd(&res[6 * (size_t)row + lane % 6], r);
+    }
+  }
+}
+
+// Residual only: R_e = sum_q D_q^T (B_q u) (C symmetric).
+__global__ void __launch_bounds__(64)
+    residual_kernel(int nelems, const int* __restrict__ conn,
+                    const double* __restrict__ xpts,
+                    const double* __restrict__ vars,
+                    const double* __restrict__ data,
+                    const int* __restrict__ perm, double* res) {
+  __shared__ ElemLDS s;
+  __shared__ double Eu[4][9];
+  const int e = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (e >= nelems) return;
+  load_element(s, e, lane, conn, xpts, vars, data);
+  __syncthreads();
+  element_geometry(s, lane);
+  element_bd(s, lane);
+  if (lane < 36) {
+    const int q = lane / 9, k = lane % 9;
+    double acc = 0.0;
+    for (int j = 0; j < 24; j++) acc += s.B[q][k][j] * s.u[j];
+    Eu[q][k] = acc;
+  }
+  __syncthreads();
+  if (lane < 24) {
+    double r = 0.0;
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int k = 0; k < 9; k++) r += s.D[q][k][lane] * Eu[q][k];
+    const int node = conn[4 * (size_t)e + lane / 6];
+    const int row = perm ? perm[node] : node;
+    unsafeAtomicAdd(&res[6 * (size_t)row + lane % 6], r);
+  }
+}
+
+// Energy U = sum_e sum_q 1/2 detXd (B_q u) . C (B_q u)
+__global__ void __launch_bounds__(64)
+    energy_kernel(int nelems, const int* __restrict__ conn,
+                  const double* __restrict__ xpts,
+                  const double* __restrict__ vars,
+                  const double* __restrict__ data, double* energy) {
+  __shared__ ElemLDS s;
+  __shared__ double Eu[4][9];
+  const int e = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (e >= nelems) return;
+  load_element(s, e, lane, conn, xpts, vars, data);
+  __syncthreads();
+  element_geometry(s, lane);
+  element_bd(s, lane);
+  if (lane < 36) {
+    const int q = lane / 9, k = lane % 9;
+    double acc = 0.0;
+    for (int j = 0; j < 24; j++) acc += s.B[q][k][j] * s.u[j];
+    Eu[q][k] = acc;
+  }
+  __syncthreads();
+  if (lane < 4) {
+    double st[9];
+    mitc4::stress(s.data, Eu[lane], st);
+    double u = 0.0;
+    for (int k = 0; k < 9; k++) u += Eu[lane][k] * st[k];
+    unsafeAtomicAdd(energy, 0.5 * s.g.scale[lane] * u);
+  }
+}
+
+}  // namespace
+}  // namespace gf
+
+extern "C" {
+
+int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
+                                   const int* conn, const double* xpts,
+                                   const double* vars, const double* data,
+                                   const int* elem_ind_map, const int* perm,
+                                   double* kvals, double* res) {
+  if (!ctx || nelems < 0 || nnodes <= 0 || !conn || !xpts || !data ||
+      !elem_ind_map || !kvals)
+    return gf::set_error(ctx, GF_ERR_INVALID, "assemble_jacobian: bad args");
+  if (nelems == 0) return GF_OK;
+  hipLaunchKernelGGL(gf::jacobian_atomic_kernel, dim3(nelems), dim3(64), 0,
+                     ctx->stream, nelems, conn, xpts, vars, data,
+                     elem_ind_map, perm, kvals, res);
+  return gf::check_launch(ctx, "jacobian_atomic_kernel");
+}
+
+int gf_assemble_residual_mitc4_f64(gf_context ctx, int nelems, int nnodes,
+                                   const int* conn, const double* xpts,
+                                   const double* vars, const double* data,
+                                   const int* perm, double* res) {
+  if (!ctx || nelems < 0 || nnodes <= 0 || !conn || !xpts || !vars ||
+      !data || !res)
+    return gf::set_error(ctx, GF_ERR_INVALID, "assemble_residual: bad args");
+  if (nelems == 0) return GF_OK;
+  hipLaunchKernelGGL(gf::residual_kernel, dim3(nelems), dim3(64), 0,
+                     ctx->stream, nelems, conn, xpts, vars, data, perm, res);
+  return gf::check_launch(ctx, "residual_kernel");
+}
+
+int gf_energy_mitc4_f64(gf_context ctx, int nelems, const int* conn,
+                        const double* xpts, const double* vars,
+                        const double* data, double* energy) {
+  if (!ctx || nelems < 0 || !conn || !xpts || !vars || !data || !energy)
+    return gf::set_error(ctx, GF_ERR_INVALID, "energy: bad args");
+  GF_HIP(ctx, hipMemsetAsync(energy, 0, sizeof(double), ctx->stream));
+  if (nelems == 0) return GF_OK;
+  hipLaunchKernelGGL(gf::energy_kernel, dim3(nelems), dim3(64), 0,
+                     ctx->stream, nelems, conn, xpts, vars, data, energy);
+  return gf::check_launch(ctx, "energy_kernel");
+}
+
+}  // extern "C"

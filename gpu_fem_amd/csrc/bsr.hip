@@ -1,0 +1,201 @@
+// BSR (6x6 blocks) kernels: boundary conditions, value copy into the
+// preconditioner pattern, vector permutation, SpMV, precision conversion.
+//
+//   bc_rows_kernel / bc_cols_kernel   bsr_mat.cuh:2-45 / :48-104
+//   bc_vec_kernel                     vec.cuh:4-13
+//   copy_pattern_kernel               bsr_mat.cuh:107-140
+//   permute_kernel                    vec.cuh:16-34
+//   spmv_kernel                       cusparseDbsrmv (gmres.cu:588-599)
+#include "common.h"
+
+namespace gf {
+namespace {
+
+__global__ void bc_rows_kernel(int nbcs, const int* __restrict__ bcs,
+                               const int* __restrict__ rowp,
+                               const int* __restrict__ cols,
+                               const int* __restrict__ perm, double* vals) {
+  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib >= nbcs) return;
+  const int dof = bcs[ib];
+  const int brow = perm[dof / 6], ir = dof % 6, grow = 6 * brow + ir;
+  for (int k = rowp[brow]; k < rowp[brow + 1]; k++) {
+    const int gc0 = 6 * cols[k];
+    double* v = vals + 36 * (size_t)k + 6 * ir;
+#pragma unroll
+    for (int c = 0; c < 6; c++) v[c] = (gc0 + c == grow) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void bc_cols_kernel(int nbcs, const int* __restrict__ bcs,
+                               const int* __restrict__ trowp,
+                               const int* __restrict__ tcols,
+                               const int* __restrict__ tmap,
+                               const int* __restrict__ perm, double* vals) {
+  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib >= nbcs) return;
+  const int dof = bcs[ib];
+  const int bcol = perm[dof / 6], ic = dof % 6, gcol = 6 * bcol + ic;
+  for (int t = trowp[bcol]; t < trowp[bcol + 1]; t++) {
+    const int gr0 = 6 * tcols[t];
+    double* v = vals + 36 * (size_t)tmap[t] + ic;
+#pragma unroll
+    for (int r = 0; r < 6; r++) v[6 * r] = (gr0 + r == gcol) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void bc_vec_kernel(int nbcs, const int* __restrict__ bcs,
+                              const int* __restrict__ perm, double* vec) {
+  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib >= nbcs) return;
+  const int dof = bcs[ib];
+  const size_t idx = perm ? 6 * (size_t)perm[dof / 6] + dof % 6 : (size_t)dof;
+  vec[idx] = 0.0;
+}
+
+// one wave per target row; lanes stride over the row's 36*nblocks values
+__global__ void __launch_bounds__(256)
+    copy_pattern_kernel(int nnodes, const int* __restrict__ rowp,
+                        const int* __restrict__ cols,
+                        const double* __restrict__ vals,
+                        const int* __restrict__ trowp,
+                        const int* __restrict__ tcols, double* tvals) {
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (row >= nnodes) return;
+  const int t0 = trowp[row], t1 = trowp[row + 1];
+  const int s0 = rowp[row], s1 = rowp[row + 1];
+  const int nval = 36 * (t1 - t0);
+  for (int v = lane; v < nval; v += 64) {
+    const int tb = t0 + v / 36, e = v % 36;
+    const int col = tcols[tb];
+    int lo = s0, hi = s1;  // lower_bound of col in the source row
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cols[mid] < col)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const double x = (lo < s1 && cols[lo] == col) ? vals[36 * (size_t)lo + e] : 0.0;
+    tvals[36 * (size_t)t0 + v] = x;
+  }
+}
+
+__global__ void permute_kernel(int nnodes, const int* __restrict__ perm,
+                               const double* __restrict__ in, double* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * (size_t)nnodes) return;
+  out[6 * (size_t)perm[i / 6] + i % 6] = in[i];
+}
+
+// six lanes per block row (one per scalar row r); ten rows per wave
+template <typename T>
+__global__ void __launch_bounds__(256)
+    spmv_kernel(int nnodes, const int* __restrict__ rowp,
+                const int* __restrict__ cols, const T* __restrict__ vals,
+                const T* __restrict__ x, T* __restrict__ y) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane >= 60) return;
+  const int row = (blockIdx.x * (blockDim.x / 64) + wave) * 10 + lane / 6;
+  const int r = lane % 6;
+  if (row >= nnodes) return;
+  T acc = 0;
+  for (int k = rowp[row]; k < rowp[row + 1]; k++) {
+    const T* a = vals + 36 * (size_t)k + 6 * r;
+    const T* xb = x + 6 * (size_t)cols[k];
+#pragma unroll
+    for (int c = 0; c < 6; c++) acc += a[c] * xb[c];
+  }
+  y[6 * (size_t)row + r] = acc;
+}
+
+__global__ void convert_kernel(size_t n, const double* __restrict__ in,
+                               float* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+}  // namespace
+}  // namespace gf
+
+extern "C" {
+
+int gf_bc_apply_mat(gf_context ctx, int nbcs, const int* bcs, const int* rowp,
+                    const int* cols, const int* trowp, const int* tcols,
+                    const int* tmap, const int* perm, double* vals) {
+  if (!ctx || nbcs < 0 || (nbcs > 0 && (!bcs || !rowp || !cols || !trowp ||
+                                        !tcols || !tmap || !perm || !vals)))
+    return gf::set_error(ctx, GF_ERR_INVALID, "bc_apply_mat: bad args");
+  if (nbcs == 0) return GF_OK;
+  const unsigned grid = gf::div_up(nbcs, 64);
+  hipLaunchKernelGGL(gf::bc_rows_kernel, dim3(grid), dim3(64), 0, ctx->stream,
+                     nbcs, bcs, rowp, cols, perm, vals);
+  GF_TRY(gf::check_launch(ctx, "bc_rows_kernel"));
+  hipLaunchKernelGGL(gf::bc_cols_kernel, dim3(grid), dim3(64), 0, ctx->stream,
+                     nbcs, bcs, trowp, tcols, tmap, perm, vals);
+  return gf::check_launch(ctx, "bc_cols_kernel");
+}
+
+int gf_bc_apply_vec(gf_context ctx, int nbcs, const int* bcs, const int* perm,
+                    double* vec) {
+  if (!ctx || nbcs < 0 || (nbcs > 0 && (!bcs || !vec)))
+    return gf::set_error(ctx, GF_ERR_INVALID, "bc_apply_vec: bad args");
+  if (nbcs == 0) return GF_OK;
+  hipLaunchKernelGGL(gf::bc_vec_kernel, dim3(gf::div_up(nbcs, 64)), dim3(64),
+                     0, ctx->stream, nbcs, bcs, perm, vec);
+  return gf::check_launch(ctx, "bc_vec_kernel");
+}
+
+int gf_bsr_copy_to_pattern(gf_context ctx, int nnodes, const int* rowp,
+                           const int* cols, const double* vals,
+                           const int* trowp, const int* tcols, double* tvals) {
+  if (!ctx || nnodes <= 0 || !rowp || !cols || !vals || !trowp || !tcols ||
+      !tvals)
+    return gf::set_error(ctx, GF_ERR_INVALID, "copy_to_pattern: bad args");
+  hipLaunchKernelGGL(gf::copy_pattern_kernel, dim3(gf::div_up(nnodes, 4)),
+                     dim3(256), 0, ctx->stream, nnodes, rowp, cols, vals,
+                     trowp, tcols, tvals);
+  return gf::check_launch(ctx, "copy_pattern_kernel");
+}
+
+int gf_permute_vec(gf_context ctx, int nnodes, const int* perm,
+                   const double* in, double* out) {
+  if (!ctx || nnodes <= 0 || !perm || !in || !out || in == out)
+    return gf::set_error(ctx, GF_ERR_INVALID, "permute_vec: bad args");
+  hipLaunchKernelGGL(gf::permute_kernel, dim3(gf::div_up(6 * (size_t)nnodes, 256)),
+                     dim3(256), 0, ctx->stream, nnodes, perm, in, out);
+  return gf::check_launch(ctx, "permute_kernel");
+}
+
+int gf_bsr_spmv_f64(gf_context ctx, int nnodes, const int* rowp,
+                    const int* cols, const double* vals, const double* x,
+                    double* y) {
+  if (!ctx || nnodes <= 0 || !rowp || !cols || !vals || !x || !y || x == y)
+    return gf::set_error(ctx, GF_ERR_INVALID, "spmv_f64: bad args");
+  hipLaunchKernelGGL(gf::spmv_kernel<double>, dim3(gf::div_up(nnodes, 40)),
+                     dim3(256), 0, ctx->stream, nnodes, rowp, cols, vals, x, y);
+  return gf::check_launch(ctx, "spmv_kernel<double>");
+}
+
+int gf_bsr_spmv_f32(gf_context ctx, int nnodes, const int* rowp,
+                    const int* cols, const float* vals, const float* x,
+                    float* y) {
+  if (!ctx || nnodes <= 0 || !rowp || !cols || !vals || !x || !y || x == y)
+    return gf::set_error(ctx, GF_ERR_INVALID, "spmv_f32: bad args");
+  hipLaunchKernelGGL(gf::spmv_kernel<float>, dim3(gf::div_up(nnodes, 40)),
+                     dim3(256), 0, ctx->stream, nnodes, rowp, cols, vals, x, y);
+  return gf::check_launch(ctx, "spmv_kernel<float>");
+}
+
+int gf_convert_f64_f32(gf_context ctx, size_t n, const double* in,
+                       float* out) {
+  if (!ctx || !in || !out)
+    return gf::set_error(ctx, GF_ERR_INVALID, "convert: bad args");
+  if (n == 0) return GF_OK;
+  hipLaunchKernelGGL(gf::convert_kernel, dim3(gf::div_up(n, 256)), dim3(256),
+                     0, ctx->stream, n, in, out);
+  return gf::check_launch(ctx, "convert_kernel");
+}
+
+}  // extern "C"

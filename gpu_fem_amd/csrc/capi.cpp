@@ -1,0 +1,358 @@
+// Context and linear-static problem entry points of the C ABI.
+//
+// gf_problem_* runs the flow of examples/iterative/gmres.cu:62-771 with the
+// reference's data conventions: symbolic work on the host (node graph ->
+// reordering -> permuted K pattern -> ILU(k) pattern -> elemIndMap ->
+// transpose map), everything uploaded once, then on the device: Jacobian
+// assembly, boundary conditions, copy into the preconditioner pattern,
+// numeric ILU, and GMRES with the permuted right-hand side
+// (solve_utils.h:4-16 bsr_pre_solve / bsr_post_solve).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "krylov.h"
+#include "symbolic.h"
+
+extern "C" {
+
+int gf_version(void) { return 1; }
+
+int gf_context_create(int device, gf_context* out) {
+  if (!out) return GF_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return GF_ERR_DEVICE;
+  auto* c = new gf_context_s;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return GF_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return GF_OK;
+}
+
+int gf_context_destroy(gf_context c) {
+  if (!c) return GF_OK;
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return GF_OK;
+}
+
+int gf_context_synchronize(gf_context c) {
+  if (!c) return GF_ERR_INVALID;
+  return gf::check_hip(c, hipStreamSynchronize(c->stream), "synchronize");
+}
+
+const char* gf_context_last_error(gf_context c) {
+  return c ? c->last_error.c_str() : "null context";
+}
+
+void* gf_context_stream(gf_context c) { return c ? (void*)c->stream : nullptr; }
+
+void gf_options_default(gf_options* o) {
+  if (!o) return;
+  o->reorder = 1;
+  o->lev_fill = 0;
+  o->restart = 50;
+  o->max_iters = 5000;
+  o->rtol = 1e-8;
+  o->precision = 0;
+}
+
+}  // extern "C"
+
+struct gf_problem_s {
+  gf_context ctx = nullptr;
+  gf_options opt{};
+  int nnodes = 0, nelems = 0, nbcs = 0;
+  int nnzb_k = 0, nnzb_m = 0;
+  std::vector<int> h_perm;
+  // device arrays
+  int *conn = nullptr, *bcs = nullptr, *perm = nullptr, *iperm = nullptr;
+  double *xpts = nullptr, *data = nullptr;
+  int *krowp = nullptr, *kcols = nullptr, *emap = nullptr;
+  int *trowp = nullptr, *tcols = nullptr, *tmap = nullptr;
+  double *kvals = nullptr, *mvals = nullptr;
+  float *kvals32 = nullptr, *mvals32 = nullptr;
+  double *rhs = nullptr, *sol = nullptr, *loads = nullptr;
+  double *work64 = nullptr, *work64b = nullptr;
+  float *t32a = nullptr, *t32b = nullptr, *t32c = nullptr;
+  gf_ilu ilu = nullptr;
+  gf::KrylovWork kw;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double t_assemble = 0, t_factor = 0;
+};
+
+namespace {
+
+template <typename T>
+int upload(gf_context ctx, T** dst, const T* src, size_t n) {
+  GF_HIP(ctx, hipMalloc((void**)dst, sizeof(T) * (n ? n : 1)));
+  if (n && src)
+    GF_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice,
+                               ctx->stream));
+  return GF_OK;
+}
+
+template <typename T>
+int dalloc(gf_context ctx, T** dst, size_t n) {
+  GF_HIP(ctx, hipMalloc((void**)dst, sizeof(T) * (n ? n : 1)));
+  return GF_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+int problem_setup(gf_problem p, const int* conn, const double* xpts,
+                  const double* data, const int* bcs) {
+  gf_context ctx = p->ctx;
+  const int n = p->nnodes, ne = p->nelems;
+  std::vector<int> rp, cl;
+  GF_TRY(gf::node_graph(ne, n, 4, conn, rp, cl));
+  p->h_perm.resize(n);
+  if (p->opt.reorder == 1) {
+    GF_TRY(gf::rcm_order(n, rp.data(), cl.data(), 0, 1, p->h_perm.data()));
+  } else if (p->opt.reorder == 2) {
+    GF_TRY(gf::amd_order(n, rp.data(), cl.data(), p->h_perm.data()));
+  } else {
+    for (int i = 0; i < n; i++) p->h_perm[i] = i;
+  }
+  std::vector<int> krp, kcl, mrp, mcl;
+  GF_TRY(gf::pattern_permute(n, rp.data(), cl.data(), p->h_perm.data(), krp,
+                             kcl));
+  GF_TRY(gf::iluk_symbolic(n, krp.data(), kcl.data(), p->opt.lev_fill, mrp,
+                           mcl, nullptr));
+  p->nnzb_k = krp[n];
+  p->nnzb_m = mrp[n];
+  std::vector<int> emap((size_t)16 * ne), tr, tc, tm;
+  GF_TRY(gf::elem_ind_map(ne, 4, conn, krp.data(), kcl.data(),
+                          p->h_perm.data(), emap.data()));
+  GF_TRY(gf::transpose_map(n, krp.data(), kcl.data(), tr, tc, tm));
+  std::vector<int> iperm(n);
+  for (int i = 0; i < n; i++) iperm[p->h_perm[i]] = i;
+  for (int i = 0; i < p->nbcs; i++)
+    if (bcs[i] < 0 || bcs[i] >= 6 * n)
+      return gf::set_error(ctx, GF_ERR_INVALID, "bc %d out of range", bcs[i]);
+
+  GF_TRY(upload(ctx, &p->conn, conn, (size_t)4 * ne));
+  GF_TRY(upload(ctx, &p->xpts, xpts, (size_t)3 * n));
+  GF_TRY(upload(ctx, &p->data, data, (size_t)4 * ne));
+  GF_TRY(upload(ctx, &p->bcs, bcs, (size_t)p->nbcs));
+  GF_TRY(upload(ctx, &p->perm, p->h_perm.data(), (size_t)n));
+  GF_TRY(upload(ctx, &p->iperm, iperm.data(), (size_t)n));
+  GF_TRY(upload(ctx, &p->krowp, krp.data(), krp.size()));
+  GF_TRY(upload(ctx, &p->kcols, kcl.data(), kcl.size()));
+  GF_TRY(upload(ctx, &p->emap, emap.data(), emap.size()));
+  GF_TRY(upload(ctx, &p->trowp, tr.data(), tr.size()));
+  GF_TRY(upload(ctx, &p->tcols, tc.data(), tc.size()));
+  GF_TRY(upload(ctx, &p->tmap, tm.data(), tm.size()));
+  GF_TRY(gf_ilu_create(ctx, n, mrp.data(), mcl.data(), &p->ilu));
+  const size_t N = 6 * (size_t)n;
+  GF_TRY(dalloc(ctx, &p->kvals, 36 * (size_t)p->nnzb_k));
+  GF_TRY(dalloc(ctx, &p->mvals, 36 * (size_t)p->nnzb_m));
+  GF_TRY(dalloc(ctx, &p->rhs, N));
+  GF_TRY(dalloc(ctx, &p->sol, N));
+  GF_TRY(dalloc(ctx, &p->loads, N));
+  GF_TRY(dalloc(ctx, &p->work64, N));
+  GF_TRY(dalloc(ctx, &p->work64b, N));
+  const bool mixed = p->opt.precision == 1;
+  if (mixed) {
+    GF_TRY(dalloc(ctx, &p->kvals32, 36 * (size_t)p->nnzb_k));
+    GF_TRY(dalloc(ctx, &p->mvals32, 36 * (size_t)p->nnzb_m));
+    GF_TRY(dalloc(ctx, &p->t32a, N));
+    GF_TRY(dalloc(ctx, &p->t32b, N));
+    GF_TRY(dalloc(ctx, &p->t32c, N));
+  }
+  GF_TRY(p->kw.init(ctx, N, p->opt.restart, mixed));
+  GF_HIP(ctx, hipEventCreate(&p->ev[0]));
+  GF_HIP(ctx, hipEventCreate(&p->ev[1]));
+  GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gf_problem_destroy(gf_problem p) {
+  if (!p) return GF_OK;
+  void* bufs[] = {p->conn,   p->bcs,   p->perm,    p->iperm,   p->xpts,
+                  p->data,   p->krowp, p->kcols,   p->emap,    p->trowp,
+                  p->tcols,  p->tmap,  p->kvals,   p->mvals,   p->kvals32,
+                  p->mvals32, p->rhs,  p->sol,     p->loads,   p->work64,
+                  p->work64b, p->t32a, p->t32b,    p->t32c};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (p->ilu) gf_ilu_destroy(p->ilu);
+  p->kw.release();
+  if (p->ev[0]) hipEventDestroy(p->ev[0]);
+  if (p->ev[1]) hipEventDestroy(p->ev[1]);
+  delete p;
+  return GF_OK;
+}
+
+int gf_problem_create(gf_context ctx, int nnodes, int nelems, const int* conn,
+                      const double* xpts, const double* data, int nbcs,
+                      const int* bcs, const gf_options* opt,
+                      gf_problem* out) {
+  if (!ctx || nnodes <= 0 || nelems <= 0 || !conn || !xpts || !data ||
+      nbcs < 0 || (nbcs > 0 && !bcs) || !out)
+    return gf::set_error(ctx, GF_ERR_INVALID, "problem_create: bad args");
+  auto* p = new gf_problem_s;
+  p->ctx = ctx;
+  if (opt)
+    p->opt = *opt;
+  else
+    gf_options_default(&p->opt);
+  if (p->opt.restart < 1 || p->opt.restart > 512 || p->opt.lev_fill < 0 ||
+      p->opt.reorder < 0 || p->opt.reorder > 2 || p->opt.precision < 0 ||
+      p->opt.precision > 1) {
+    delete p;
+    return gf::set_error(ctx, GF_ERR_INVALID, "problem_create: bad options");
+  }
+  p->nnodes = nnodes;
+  p->nelems = nelems;
+  p->nbcs = nbcs;
+  int st = problem_setup(p, conn, xpts, data, bcs);
+  if (st != GF_OK) {
+    gf_problem_destroy(p);
+    return st;
+  }
+  *out = p;
+  return GF_OK;
+}
+
+int gf_problem_assemble(gf_problem p) {
+  if (!p) return GF_ERR_INVALID;
+  gf_context ctx = p->ctx;
+  GF_HIP(ctx, hipEventRecord(p->ev[0], ctx->stream));
+  GF_HIP(ctx, hipMemsetAsync(p->kvals, 0, sizeof(double) * 36 * p->nnzb_k,
+                             ctx->stream));
+  GF_TRY(gf_assemble_jacobian_mitc4_f64(ctx, p->nelems, p->nnodes, p->conn,
+                                        p->xpts, nullptr, p->data, p->emap,
+                                        p->perm, p->kvals, nullptr));
+  GF_TRY(gf_bc_apply_mat(ctx, p->nbcs, p->bcs, p->krowp, p->kcols, p->trowp,
+                         p->tcols, p->tmap, p->perm, p->kvals));
+  GF_TRY(gf_bsr_copy_to_pattern(ctx, p->nnodes, p->krowp, p->kcols, p->kvals,
+                                gf_ilu_device_rowp(p->ilu),
+                                gf_ilu_device_cols(p->ilu), p->mvals));
+  GF_HIP(ctx, hipEventRecord(p->ev[1], ctx->stream));
+  GF_HIP(ctx, hipEventSynchronize(p->ev[1]));
+  p->t_assemble = 1e-3 * elapsed(p->ev[0], p->ev[1]);
+  return GF_OK;
+}
+
+int gf_problem_factor(gf_problem p) {
+  if (!p) return GF_ERR_INVALID;
+  gf_context ctx = p->ctx;
+  GF_HIP(ctx, hipEventRecord(p->ev[0], ctx->stream));
+  int zp = -1;
+  if (p->opt.precision == 1) {
+    GF_TRY(gf_convert_f64_f32(ctx, 36 * (size_t)p->nnzb_m, p->mvals,
+                              p->mvals32));
+    GF_TRY(gf_convert_f64_f32(ctx, 36 * (size_t)p->nnzb_k, p->kvals,
+                              p->kvals32));
+    GF_TRY(gf_bsr_ilu_factor_f32(p->ilu, p->mvals32, &zp));
+  } else {
+    GF_TRY(gf_bsr_ilu_factor_f64(p->ilu, p->mvals, &zp));
+  }
+  GF_HIP(ctx, hipEventRecord(p->ev[1], ctx->stream));
+  GF_HIP(ctx, hipEventSynchronize(p->ev[1]));
+  p->t_factor = 1e-3 * elapsed(p->ev[0], p->ev[1]);
+  return GF_OK;
+}
+
+int gf_problem_solve(gf_problem p, const double* loads, double* x,
+                     int is_device, gf_solve_info* info) {
+  if (!p || !loads || !x) return GF_ERR_INVALID;
+  gf_context ctx = p->ctx;
+  hipStream_t s = ctx->stream;
+  const size_t N = 6 * (size_t)p->nnodes;
+  GF_HIP(ctx, hipEventRecord(p->ev[0], s));
+  GF_HIP(ctx, hipMemcpyAsync(p->loads, loads, sizeof(double) * N,
+                             is_device ? hipMemcpyDeviceToDevice
+                                       : hipMemcpyHostToDevice,
+                             s));
+  GF_TRY(gf_bc_apply_vec(ctx, p->nbcs, p->bcs, nullptr, p->loads));
+  GF_TRY(gf_permute_vec(ctx, p->nnodes, p->perm, p->loads, p->rhs));
+  GF_HIP(ctx, hipMemsetAsync(p->sol, 0, sizeof(double) * N, s));
+  gf::Operator op;
+  op.ctx = ctx;
+  op.n = p->nnodes;
+  op.krowp = p->krowp;
+  op.kcols = p->kcols;
+  op.kvals = p->kvals;
+  op.kvals32 = p->kvals32;
+  op.ilu = p->ilu;
+  op.mlu = p->mvals;
+  op.mlu32 = p->mvals32;
+  op.work64 = p->work64;
+  op.work64b = p->work64b;
+  op.t32a = p->t32a;
+  op.t32b = p->t32b;
+  op.t32c = p->t32c;
+  gf::GmresResult gr;
+  const int st = gf::gmres_solve(p->kw, op, p->rhs, p->sol, p->opt.max_iters,
+                                 p->opt.rtol, p->opt.precision == 1, &gr);
+  // x = inverse permutation of sol: x[old] = sol[perm[old]]
+  GF_TRY(gf_permute_vec(ctx, p->nnodes, p->iperm, p->sol, p->work64));
+  GF_HIP(ctx, hipMemcpyAsync(x, p->work64, sizeof(double) * N,
+                             is_device ? hipMemcpyDeviceToDevice
+                                       : hipMemcpyDeviceToHost,
+                             s));
+  GF_HIP(ctx, hipEventRecord(p->ev[1], s));
+  GF_HIP(ctx, hipEventSynchronize(p->ev[1]));
+  if (info) {
+    info->status = st;
+    info->iterations = gr.iterations;
+    info->outer = gr.outer;
+    info->rel_resid = gr.rel_resid;
+    info->t_assemble = p->t_assemble;
+    info->t_factor = p->t_factor;
+    info->t_solve = 1e-3 * elapsed(p->ev[0], p->ev[1]);
+  }
+  return st;
+}
+
+int gf_problem_sizes(gf_problem p, int* nnodes, int* nnzb_k, int* nnzb_m) {
+  if (!p) return GF_ERR_INVALID;
+  if (nnodes) *nnodes = p->nnodes;
+  if (nnzb_k) *nnzb_k = p->nnzb_k;
+  if (nnzb_m) *nnzb_m = p->nnzb_m;
+  return GF_OK;
+}
+
+int gf_problem_device_arrays(gf_problem p, const int** krowp,
+                             const int** kcols, double** kvals,
+                             const int** mrowp, const int** mcols,
+                             double** mvals, const int** perm) {
+  if (!p) return GF_ERR_INVALID;
+  if (krowp) *krowp = p->krowp;
+  if (kcols) *kcols = p->kcols;
+  if (kvals) *kvals = p->kvals;
+  if (mrowp) *mrowp = gf_ilu_device_rowp(p->ilu);
+  if (mcols) *mcols = gf_ilu_device_cols(p->ilu);
+  if (mvals) *mvals = p->mvals;
+  if (perm) *perm = p->perm;
+  return GF_OK;
+}
+
+int gf_problem_host_perm(gf_problem p, int* perm) {
+  if (!p || !perm) return GF_ERR_INVALID;
+  std::memcpy(perm, p->h_perm.data(), sizeof(int) * p->nnodes);
+  return GF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,326 @@
+// Block ILU numeric factorisation and the L/U triangular sweeps on a 6x6 BSR
+// pattern (replaces cusparseXbsrilu02 and cusparseXbsrsv2,
+// examples/iterative/gmres.cu:374-465 and :609-625).
+//
+// Semantics: in-place incomplete LU on the given block pattern, equal to
+// scalar ILU on the block-expanded pattern: strictly-lower blocks hold L_ik,
+// the diagonal block holds unit-lower L_ii below and U_ii on/above its
+// diagonal, upper blocks hold U_ij.  The L sweep has a unit diagonal, the U
+// sweep a non-unit one.
+//
+// Schedule: rows are grouped by dependency level on the host (the lower
+// levels of the factorisation equal those of the L sweep).  Each level is
+// one launch; rows inside a level are independent.
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "symbolic.h"
+
+struct gf_ilu_s {
+  gf_context ctx = nullptr;
+  int n = 0, nnzb = 0;
+  int *rowp = nullptr, *cols = nullptr, *diag = nullptr;
+  int *lrows = nullptr, *urows = nullptr;  // rows ordered by level
+  std::vector<int> lptr, uptr;             // level offsets (host)
+  int* pivot = nullptr;                    // first zero-pivot row (device)
+  int max_row_len = 0;
+};
+
+namespace gf {
+namespace {
+
+constexpr int kMaxRowBlocks = 1024;
+
+// One 256-thread workgroup per row of the current level.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    ilu_row_kernel(const int* __restrict__ level_rows, int nrows,
+                   const int* __restrict__ rowp, const int* __restrict__ cols,
+                   const int* __restrict__ diag, T* vals, int* pivot) {
+  __shared__ int pos[kMaxRowBlocks];
+  const int li = blockIdx.x;
+  if (li >= nrows) return;
+  const int i = level_rows[li];
+  const int tid = threadIdx.x;
+  const int r0 = rowp[i], r1 = rowp[i + 1], di = diag[i];
+  for (int kk = r0; kk < di; kk++) {
+    const int k = cols[kk];
+    T* Aik = vals + 36 * (size_t)kk;
+    const T* Ukk = vals + 36 * (size_t)diag[k];
+    // L_ik = A_ik U_kk^-1 (row r by forward substitution over columns)
+    if (tid < 6) {
+      const int r = tid;
+      T x[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        T s = Aik[6 * r + c];
+#pragma unroll
+        for (int m = 0; m < c; m++) s -= x[m] * Ukk[6 * m + c];
+        x[c] = s / Ukk[6 * c + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 6; c++) Aik[6 * r + c] = x[c];
+    }
+    // positions in row i of the U part of row k
+    const int u0 = diag[k] + 1, u1 = rowp[k + 1];
+    const int nu = u1 - u0;
+    for (int t = tid; t < nu; t += blockDim.x) {
+      const int j = cols[u0 + t];
+      int lo = kk + 1, hi = r1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cols[mid] < j)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      pos[t] = (lo < r1 && cols[lo] == j) ? lo : -1;
+    }
+    __syncthreads();
+    // A_ij -= L_ik U_kj
+    for (int w = tid; w < 36 * nu; w += blockDim.x) {
+      const int t = w / 36, e = w % 36, r = e / 6, c = e % 6;
+      const int p = pos[t];
+      if (p < 0) continue;
+      const T* Ukj = vals + 36 * (size_t)(u0 + t);
+      T s = 0;
+#pragma unroll
+      for (int m = 0; m < 6; m++) s += Aik[6 * r + m] * Ukj[6 * m + c];
+      vals[36 * (size_t)p + e] -= s;
+    }
+    __syncthreads();
+  }
+  // factor the diagonal block (no pivoting)
+  T* D = vals + 36 * (size_t)di;
+  if (tid == 0) {
+    for (int c = 0; c < 6; c++) {
+      const T piv = D[6 * c + c];
+      if (piv == T(0)) {
+        atomicMin(pivot, i);
+        break;
+      }
+      for (int r = c + 1; r < 6; r++) {
+        const T l = D[6 * r + c] / piv;
+        D[6 * r + c] = l;
+        for (int m = c + 1; m < 6; m++) D[6 * r + m] -= l * D[6 * c + m];
+      }
+    }
+  }
+  __syncthreads();
+  // U_ij = L_ii^-1 A_ij for the blocks right of the diagonal
+  const int nr = r1 - (di + 1);
+  for (int w = tid; w < 6 * nr; w += blockDim.x) {
+    const int t = w / 6, c = w % 6;
+    T* A = vals + 36 * (size_t)(di + 1 + t);
+    T x[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      T s = A[6 * r + c];
+#pragma unroll
+      for (int m = 0; m < r; m++) s -= D[6 * r + m] * x[m];
+      x[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; r++) A[6 * r + c] = x[r];
+  }
+}
+
+// Six lanes per row (lane = scalar row r), ten rows per wave.
+template <typename T, bool kLower>
+__global__ void __launch_bounds__(256)
+    trsv_level_kernel(const int* __restrict__ level_rows, int nrows,
+                      const int* __restrict__ rowp,
+                      const int* __restrict__ cols,
+                      const int* __restrict__ diag, const T* __restrict__ lu,
+                      const T* __restrict__ b, T* __restrict__ x) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int li = (blockIdx.x * (blockDim.x / 64) + wave) * 10 + lane / 6;
+  const int r = lane % 6;
+  const bool active = lane < 60 && li < nrows;
+  const int i = active ? level_rows[li] : 0;
+  T s = 0;
+  int di = 0;
+  if (active) {
+    di = diag[i];
+    s = b[6 * (size_t)i + r];
+    const int k0 = kLower ? rowp[i] : di + 1;
+    const int k1 = kLower ? di : rowp[i + 1];
+    for (int k = k0; k < k1; k++) {
+      const T* a = lu + 36 * (size_t)k + 6 * r;
+      const T* xk = x + 6 * (size_t)cols[k];
+#pragma unroll
+      for (int c = 0; c < 6; c++) s -= a[c] * xk[c];
+    }
+  }
+  // diagonal block solve across the six lanes of the row
+  const int base = lane - r;  // first lane of this row's group
+  const T* D = lu + 36 * (size_t)di + 6 * r;
+  if (kLower) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const T yc = __shfl(s, base + c, 64);
+      if (active && r > c) s -= D[c] * yc;
+    }
+  } else {
+#pragma unroll
+    for (int c = 5; c >= 0; c--) {
+      if (active && r == c) s /= D[c];
+      const T yc = __shfl(s, base + c, 64);
+      if (active && r < c) s -= D[c] * yc;
+    }
+  }
+  if (active) x[6 * (size_t)i + r] = s;
+}
+
+}  // namespace
+}  // namespace gf
+
+extern "C" {
+
+int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
+                  gf_ilu* out) {
+  if (!ctx || n <= 0 || !rowp || !cols || !out)
+    return gf::set_error(ctx, GF_ERR_INVALID, "ilu_create: bad args");
+  std::vector<int> diag(n, -1);
+  int maxlen = 0;
+  for (int i = 0; i < n; i++) {
+    maxlen = std::max(maxlen, rowp[i + 1] - rowp[i]);
+    for (int k = rowp[i]; k < rowp[i + 1]; k++) {
+      if (cols[k] < 0 || cols[k] >= n || (k > rowp[i] && cols[k] <= cols[k - 1]))
+        return gf::set_error(ctx, GF_ERR_INVALID,
+                             "ilu_create: row %d not sorted/in range", i);
+      if (cols[k] == i) diag[i] = k;
+    }
+    if (diag[i] < 0)
+      return gf::set_error(ctx, GF_ERR_INVALID,
+                           "ilu_create: row %d has no diagonal block", i);
+  }
+  if (maxlen > gf::kMaxRowBlocks)
+    return gf::set_error(ctx, GF_ERR_INVALID,
+                         "ilu_create: row length %d exceeds %d blocks", maxlen,
+                         gf::kMaxRowBlocks);
+  auto* h = new gf_ilu_s;
+  h->ctx = ctx;
+  h->n = n;
+  h->nnzb = rowp[n];
+  h->max_row_len = maxlen;
+  std::vector<int> lrows, urows;
+  gf::level_schedule(n, rowp, cols, true, h->lptr, lrows);
+  gf::level_schedule(n, rowp, cols, false, h->uptr, urows);
+  auto up = [&](int** dst, const int* src, size_t cnt) -> int {
+    GF_HIP(ctx, hipMalloc(dst, sizeof(int) * (cnt ? cnt : 1)));
+    GF_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(int) * cnt,
+                               hipMemcpyHostToDevice, ctx->stream));
+    return GF_OK;
+  };
+  int st = GF_OK;
+  if ((st = up(&h->rowp, rowp, n + 1)) || (st = up(&h->cols, cols, rowp[n])) ||
+      (st = up(&h->diag, diag.data(), n)) ||
+      (st = up(&h->lrows, lrows.data(), n)) ||
+      (st = up(&h->urows, urows.data(), n)) ||
+      (st = gf::check_hip(ctx, hipMalloc(&h->pivot, sizeof(int)), "malloc"))) {
+    gf_ilu_destroy(h);
+    return st;
+  }
+  st = gf::check_hip(ctx, hipStreamSynchronize(ctx->stream), "ilu upload");
+  if (st) {
+    gf_ilu_destroy(h);
+    return st;
+  }
+  *out = h;
+  return GF_OK;
+}
+
+int gf_ilu_destroy(gf_ilu h) {
+  if (!h) return GF_OK;
+  hipFree(h->rowp);
+  hipFree(h->cols);
+  hipFree(h->diag);
+  hipFree(h->lrows);
+  hipFree(h->urows);
+  hipFree(h->pivot);
+  delete h;
+  return GF_OK;
+}
+
+int gf_ilu_info(gf_ilu h, int* nnzb, int* nl, int* nu) {
+  if (!h) return GF_ERR_INVALID;
+  if (nnzb) *nnzb = h->nnzb;
+  if (nl) *nl = (int)h->lptr.size() - 1;
+  if (nu) *nu = (int)h->uptr.size() - 1;
+  return GF_OK;
+}
+const int* gf_ilu_device_rowp(gf_ilu h) { return h ? h->rowp : nullptr; }
+const int* gf_ilu_device_cols(gf_ilu h) { return h ? h->cols : nullptr; }
+
+}  // extern "C"
+
+namespace {
+template <typename T>
+int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
+  if (!h || !vals) return GF_ERR_INVALID;
+  gf_context ctx = h->ctx;
+  const int big = 0x7fffffff;
+  GF_HIP(ctx, hipMemcpyAsync(h->pivot, &big, sizeof(int),
+                             hipMemcpyHostToDevice, ctx->stream));
+  const int nl = (int)h->lptr.size() - 1;
+  for (int l = 0; l < nl; l++) {
+    const int cnt = h->lptr[l + 1] - h->lptr[l];
+    hipLaunchKernelGGL(gf::ilu_row_kernel<T>, dim3(cnt), dim3(256), 0,
+                       ctx->stream, h->lrows + h->lptr[l], cnt, h->rowp,
+                       h->cols, h->diag, vals, h->pivot);
+  }
+  GF_TRY(gf::check_launch(ctx, "ilu_row_kernel"));
+  int piv = big;
+  GF_HIP(ctx, hipMemcpyAsync(&piv, h->pivot, sizeof(int),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (zero_pivot) *zero_pivot = piv == big ? -1 : piv;
+  if (piv != big)
+    return gf::set_error(ctx, GF_ERR_ZERO_PIVOT,
+                         "ILU: zero pivot in diagonal block of row %d", piv);
+  return GF_OK;
+}
+
+template <typename T>
+int ilu_apply(gf_ilu h, const T* lu, const T* b, T* tmp, T* x) {
+  if (!h || !lu || !b || !tmp || !x || b == tmp || tmp == x)
+    return GF_ERR_INVALID;
+  gf_context ctx = h->ctx;
+  const int nl = (int)h->lptr.size() - 1;
+  for (int l = 0; l < nl; l++) {
+    const int cnt = h->lptr[l + 1] - h->lptr[l];
+    hipLaunchKernelGGL((gf::trsv_level_kernel<T, true>),
+                       dim3(gf::div_up(cnt, 40)), dim3(256), 0, ctx->stream,
+                       h->lrows + h->lptr[l], cnt, h->rowp, h->cols, h->diag,
+                       lu, b, tmp);
+  }
+  const int nu = (int)h->uptr.size() - 1;
+  for (int l = 0; l < nu; l++) {
+    const int cnt = h->uptr[l + 1] - h->uptr[l];
+    hipLaunchKernelGGL((gf::trsv_level_kernel<T, false>),
+                       dim3(gf::div_up(cnt, 40)), dim3(256), 0, ctx->stream,
+                       h->urows + h->uptr[l], cnt, h->rowp, h->cols, h->diag,
+                       lu, tmp, x);
+  }
+  return gf::check_launch(ctx, "trsv_level_kernel");
+}
+}  // namespace
+
+extern "C" {
+int gf_bsr_ilu_factor_f64(gf_ilu h, double* vals, int* zp) {
+  return ilu_factor<double>(h, vals, zp);
+}
+int gf_bsr_ilu_factor_f32(gf_ilu h, float* vals, int* zp) {
+  return ilu_factor<float>(h, vals, zp);
+}
+int gf_bsr_ilu_apply_f64(gf_ilu h, const double* lu, const double* b,
+                         double* tmp, double* x) {
+  return ilu_apply<double>(h, lu, b, tmp, x);
+}
+int gf_bsr_ilu_apply_f32(gf_ilu h, const float* lu, const float* b,
+                         float* tmp, float* x) {
+  return ilu_apply<float>(h, lu, b, tmp, x);
+}
+}  // extern "C"

@@ -1,0 +1,53 @@
+// GMRES driver interfaces (see krylov.hip).
+#pragma once
+#include <cstddef>
+
+#include "common.h"
+
+namespace gf {
+
+// Device workspace for GMRES(m) on N = 6 * nnodes unknowns.
+struct KrylovWork {
+  gf_context ctx = nullptr;
+  size_t N = 0;
+  int m = 0;
+  double *V = nullptr, *w = nullptr, *z = nullptr, *r = nullptr,
+         *tmp = nullptr;
+  double *h = nullptr, *h2 = nullptr, *nrm = nullptr, *y = nullptr;
+  double *state = nullptr, *partial = nullptr, *scal = nullptr;
+  float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
+  double* host_scal = nullptr;  // pinned
+  int init(gf_context c, size_t N, int m, bool mixed);
+  void release();
+  int dots(const double* V, size_t ldv, int nvec, const double* x, double* out,
+           int mode);
+};
+
+// The linear operator A (BSR, FP64 and optional FP32 copy) and the ILU
+// preconditioner M (factored values in FP64 and/or FP32).
+struct Operator {
+  gf_context ctx = nullptr;
+  int n = 0;
+  const int *krowp = nullptr, *kcols = nullptr;
+  const double* kvals = nullptr;
+  const float* kvals32 = nullptr;
+  gf_ilu ilu = nullptr;
+  const double* mlu = nullptr;
+  const float* mlu32 = nullptr;
+  double *work64 = nullptr, *work64b = nullptr;
+  float *t32a = nullptr, *t32b = nullptr, *t32c = nullptr;
+  int apply_a64(const double* in, double* out) const;
+  int apply_prec(const double* in, double* out, bool mixed) const;
+  int apply_prec_a(const double* in, double* out, bool mixed) const;
+};
+
+struct GmresResult {
+  int iterations = 0;
+  int outer = 0;
+  double rel_resid = 0.0;
+};
+
+int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
+                int max_iters, double rtol, bool mixed, GmresResult* res);
+
+}  // namespace gf

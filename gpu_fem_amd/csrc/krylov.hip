@@ -1,0 +1,361 @@
+// Krylov building blocks and the restarted, right-preconditioned GMRES(m)
+// driver (FP64) plus its mixed-precision GMRES-IR variant.
+//
+// Replaces the host-driven loop of examples/iterative/gmres.cu:472-771 and the
+// cuBLAS level-1 calls it makes (Dnrm2/Ddot/Daxpy/Dscal/Dtrsv, :540-767).  All
+// reductions are two-pass and deterministic; the Hessenberg column, Givens
+// rotations and least-squares solve stay on the device, and the host reads one
+// scalar (the residual estimate) per Arnoldi step.
+//
+// Orthogonalisation is classical Gram-Schmidt with one re-orthogonalisation
+// pass (CGS2): two multi-dot launches and two updates per step instead of the
+// (j+1) serial dot/axpy pairs of modified Gram-Schmidt.
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "krylov.h"
+
+namespace gf {
+namespace {
+
+constexpr int kDotBlocks = 512;
+constexpr int kDotThreads = 256;
+constexpr int kDotChunk = 8;
+
+// partial[b * nvec + i] = sum over block b's slice of V_i . w
+__global__ void __launch_bounds__(kDotThreads)
+    multi_dot_partial(size_t N, int nvec, const double* __restrict__ V,
+                      size_t ldv, const double* __restrict__ w,
+                      double* __restrict__ partial) {
+  __shared__ double red[kDotThreads / 64][kDotChunk];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  for (int i0 = 0; i0 < nvec; i0 += kDotChunk) {
+    double acc[kDotChunk];
+#pragma unroll
+    for (int c = 0; c < kDotChunk; c++) acc[c] = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+         t += (size_t)gridDim.x * blockDim.x) {
+      const double wt = w[t];
+#pragma unroll
+      for (int c = 0; c < kDotChunk; c++)
+        if (i0 + c < nvec) acc[c] += V[(size_t)(i0 + c) * ldv + t] * wt;
+    }
+#pragma unroll
+    for (int c = 0; c < kDotChunk; c++) {
+      double v = acc[c];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) red[wave][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDotChunk && i0 + (int)threadIdx.x < nvec) {
+      double v = 0.0;
+      for (int wv = 0; wv < kDotThreads / 64; wv++) v += red[wv][threadIdx.x];
+      partial[(size_t)blockIdx.x * nvec + i0 + threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// out[i] (+)= sum_b partial[b*nvec+i]; mode 0: store, 1: add, 2: store sqrt
+__global__ void multi_dot_final(int nblocks, int nvec,
+                                const double* __restrict__ partial,
+                                double* __restrict__ out, int mode) {
+  const int i = threadIdx.x;
+  if (i >= nvec) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * nvec + i];
+  if (mode == 1)
+    out[i] += s;
+  else if (mode == 2)
+    out[i] = sqrt(s);
+  else
+    out[i] = s;
+}
+
+// w -= sum_i h[i] V_i
+__global__ void update_w(size_t N, int nvec, const double* __restrict__ V,
+                         size_t ldv, const double* __restrict__ h,
+                         double* __restrict__ w) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  double s = w[t];
+  for (int i = 0; i < nvec; i++) s -= h[i] * V[(size_t)i * ldv + t];
+  w[t] = s;
+}
+
+// u = sum_i y[i] V_i
+__global__ void lincomb(size_t N, int nvec, const double* __restrict__ V,
+                        size_t ldv, const double* __restrict__ y,
+                        double* __restrict__ u) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  double s = 0.0;
+  for (int i = 0; i < nvec; i++) s += y[i] * V[(size_t)i * ldv + t];
+  u[t] = s;
+}
+
+// v = w / *nrm (0 if nrm == 0)
+__global__ void scale_by_inv(size_t N, const double* __restrict__ w,
+                             const double* __restrict__ nrm,
+                             double* __restrict__ v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const double d = *nrm;
+  v[t] = d > 0.0 ? w[t] / d : 0.0;
+}
+
+// r = b - r
+__global__ void sub_from(size_t N, const double* __restrict__ b,
+                         double* __restrict__ r) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N) r[t] = b[t] - r[t];
+}
+
+__global__ void axpy_k(size_t N, double a, const double* __restrict__ x,
+                       double* __restrict__ y) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N) y[t] += a * x[t];
+}
+
+__global__ void to_f32(size_t N, const double* __restrict__ in,
+                       float* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N) out[t] = (float)in[t];
+}
+__global__ void to_f64(size_t N, const float* __restrict__ in,
+                       double* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N) out[t] = (double)in[t];
+}
+
+// Arnoldi column j: H[0..j, j] = h, H[j+1, j] = hn; apply the previous
+// rotations, form the new one, update g; out = |g[j+1]|, g_j1 stored.
+// State layout in `st`: H (m+1)*m | cs m | sn m | g m+1
+__global__ void givens_step(int j, int m, const double* __restrict__ h,
+                            const double* __restrict__ hn,
+                            double* __restrict__ st, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* H = st;
+  double* cs = st + (size_t)(m + 1) * m;
+  double* sn = cs + m;
+  double* g = sn + m;
+  for (int i = 0; i <= j; i++) H[i * m + j] = h[i];
+  H[(j + 1) * m + j] = *hn;
+  for (int i = 0; i < j; i++) {
+    const double a = H[i * m + j], c = H[(i + 1) * m + j];
+    H[i * m + j] = cs[i] * a + sn[i] * c;
+    H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+  }
+  const double a = H[j * m + j], c = H[(j + 1) * m + j];
+  const double den = sqrt(a * a + c * c);
+  cs[j] = den > 0.0 ? a / den : 1.0;
+  sn[j] = den > 0.0 ? c / den : 0.0;
+  H[j * m + j] = cs[j] * a + sn[j] * c;
+  H[(j + 1) * m + j] = 0.0;
+  g[j + 1] = -sn[j] * g[j];
+  g[j] = cs[j] * g[j];
+  out[0] = fabs(g[j + 1]);
+  out[1] = *hn;
+}
+
+__global__ void init_g(int m, const double* __restrict__ beta,
+                       double* __restrict__ st) {
+  double* g = st + (size_t)(m + 1) * m + 2 * m;
+  const int i = threadIdx.x;
+  if (i <= m) g[i] = (i == 0) ? *beta : 0.0;
+}
+
+// y = H[0:k,0:k]^-1 g[0:k]
+__global__ void solve_upper(int k, int m, const double* __restrict__ st,
+                            double* __restrict__ y) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double* H = st;
+  const double* g = st + (size_t)(m + 1) * m + 2 * m;
+  for (int i = k - 1; i >= 0; i--) {
+    double s = g[i];
+    for (int t = i + 1; t < k; t++) s -= H[i * m + t] * y[t];
+    y[i] = s / H[i * m + i];
+  }
+}
+
+inline dim3 grid1(size_t N) { return dim3(div_up(N, 256)); }
+
+}  // namespace
+
+int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
+  ctx = c;
+  N = N_;
+  m = m_;
+  auto dm = [&](void** p, size_t bytes) {
+    return check_hip(ctx, hipMalloc(p, bytes ? bytes : 8), "krylov malloc");
+  };
+  GF_TRY(dm((void**)&V, sizeof(double) * N * (m + 1)));
+  GF_TRY(dm((void**)&w, sizeof(double) * N));
+  GF_TRY(dm((void**)&z, sizeof(double) * N));
+  GF_TRY(dm((void**)&r, sizeof(double) * N));
+  GF_TRY(dm((void**)&tmp, sizeof(double) * N));
+  GF_TRY(dm((void**)&h, sizeof(double) * (m + 2)));
+  GF_TRY(dm((void**)&h2, sizeof(double) * (m + 2)));
+  GF_TRY(dm((void**)&nrm, sizeof(double) * 2));
+  GF_TRY(dm((void**)&y, sizeof(double) * (m + 1)));
+  GF_TRY(dm((void**)&state, sizeof(double) * ((m + 1) * m + 3 * m + 2)));
+  GF_TRY(dm((void**)&partial, sizeof(double) * kDotBlocks * (m + 2)));
+  GF_TRY(dm((void**)&scal, sizeof(double) * 4));
+  if (mixed) {
+    GF_TRY(dm((void**)&a32, sizeof(float) * N));
+    GF_TRY(dm((void**)&b32, sizeof(float) * N));
+    GF_TRY(dm((void**)&c32, sizeof(float) * N));
+  }
+  GF_TRY(check_hip(ctx, hipHostMalloc((void**)&host_scal, sizeof(double) * 4),
+                   "hipHostMalloc"));
+  return GF_OK;
+}
+
+void KrylovWork::release() {
+  hipFree(V);
+  hipFree(w);
+  hipFree(z);
+  hipFree(r);
+  hipFree(tmp);
+  hipFree(h);
+  hipFree(h2);
+  hipFree(nrm);
+  hipFree(y);
+  hipFree(state);
+  hipFree(partial);
+  hipFree(scal);
+  hipFree(a32);
+  hipFree(b32);
+  hipFree(c32);
+  if (host_scal) hipHostFree(host_scal);
+  *this = KrylovWork();
+}
+
+// out[0..nvec) = V_i . w  (mode: 0 store, 1 add, 2 sqrt)
+int KrylovWork::dots(const double* Vb, size_t ldv, int nvec, const double* x,
+                     double* out, int mode) {
+  hipLaunchKernelGGL(multi_dot_partial, dim3(kDotBlocks), dim3(kDotThreads), 0,
+                     ctx->stream, N, nvec, Vb, ldv, x, partial);
+  hipLaunchKernelGGL(multi_dot_final, dim3(1), dim3(64 * div_up(nvec, 64)), 0,
+                     ctx->stream, (int)kDotBlocks, nvec, partial, out, mode);
+  return check_launch(ctx, "multi_dot");
+}
+
+int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
+                int max_iters, double rtol, bool mixed, GmresResult* res) {
+  gf_context ctx = W.ctx;
+  hipStream_t s = ctx->stream;
+  const size_t N = W.N;
+  const int m = W.m;
+  // ||b||
+  GF_TRY(W.dots(b, N, 1, b, W.scal, 2));
+  GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.scal, sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+  GF_HIP(ctx, hipStreamSynchronize(s));
+  double bnorm = W.host_scal[0];
+  if (bnorm == 0.0) bnorm = 1.0;
+  int it = 0, outer = 0, status = GF_ERR_NO_CONVERGENCE;
+  double true_rel = 0.0;
+  for (;;) {
+    // r = b - A x (FP64 always)
+    GF_TRY(op.apply_a64(x, W.r));
+    hipLaunchKernelGGL(sub_from, grid1(N), dim3(256), 0, s, N, b, W.r);
+    GF_TRY(W.dots(W.r, N, 1, W.r, W.nrm, 2));
+    GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.nrm, sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    GF_HIP(ctx, hipStreamSynchronize(s));
+    const double beta = W.host_scal[0];
+    true_rel = beta / bnorm;
+    if (!std::isfinite(beta)) {
+      status = set_error(ctx, GF_ERR_BREAKDOWN, "GMRES: non-finite residual");
+      break;
+    }
+    if (true_rel <= rtol) {
+      status = GF_OK;
+      break;
+    }
+    if (it >= max_iters) break;
+    outer++;
+    const double tol_abs =
+        mixed ? std::max(1e-5 * beta, 0.5 * rtol * bnorm) : rtol * bnorm;
+    hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.r, W.nrm,
+                       W.V);
+    hipLaunchKernelGGL(init_g, dim3(1), dim3(64 * div_up(m + 1, 64)), 0, s, m,
+                       W.nrm, W.state);
+    int j = 0;
+    while (j < m && it < max_iters) {
+      double* vj = W.V + (size_t)j * N;
+      GF_TRY(op.apply_prec_a(vj, W.w, mixed));  // w = A M^-1 v_j
+      it++;
+      // CGS2
+      GF_TRY(W.dots(W.V, N, j + 1, W.w, W.h, 0));
+      hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, j + 1, W.V, N,
+                         W.h, W.w);
+      GF_TRY(W.dots(W.V, N, j + 1, W.w, W.h2, 0));
+      hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, j + 1, W.V, N,
+                         W.h2, W.w);
+      hipLaunchKernelGGL(axpy_k, dim3(1), dim3(64 * div_up(j + 1, 64)), 0, s,
+                         (size_t)(j + 1), 1.0, W.h2, W.h);
+      GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm, 2));
+      hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.w,
+                         W.nrm, W.V + (size_t)(j + 1) * N);
+      hipLaunchKernelGGL(givens_step, dim3(1), dim3(64), 0, s, j, m, W.h,
+                         W.nrm, W.state, W.scal);
+      GF_TRY(check_launch(ctx, "arnoldi step"));
+      GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.scal, 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+      GF_HIP(ctx, hipStreamSynchronize(s));
+      j++;
+      const double est = W.host_scal[0], hn = W.host_scal[1];
+      if (!std::isfinite(est)) break;
+      if (est <= tol_abs || hn == 0.0) break;
+    }
+    // x += M^-1 V y
+    hipLaunchKernelGGL(solve_upper, dim3(1), dim3(64), 0, s, j, m, W.state,
+                       W.y);
+    hipLaunchKernelGGL(lincomb, grid1(N), dim3(256), 0, s, N, j, W.V, N, W.y,
+                       W.tmp);
+    GF_TRY(op.apply_prec(W.tmp, W.z, mixed));
+    hipLaunchKernelGGL(axpy_k, grid1(N), dim3(256), 0, s, N, 1.0, W.z, x);
+    GF_TRY(check_launch(ctx, "gmres update"));
+  }
+  if (res) {
+    res->iterations = it;
+    res->outer = outer;
+    res->rel_resid = true_rel;
+  }
+  if (status == GF_ERR_NO_CONVERGENCE)
+    set_error(ctx, status, "GMRES: %d iterations, rel. residual %.3e > %.1e",
+              it, true_rel, rtol);
+  return status;
+}
+
+// ---------------------------------------------------------------- operator
+int Operator::apply_a64(const double* in, double* out) const {
+  return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, in, out);
+}
+
+int Operator::apply_prec(const double* in, double* out, bool mixed) const {
+  const size_t N = 6 * (size_t)n;
+  if (!mixed) return gf_bsr_ilu_apply_f64(ilu, mlu, in, work64, out);
+  hipLaunchKernelGGL(to_f32, grid1(N), dim3(256), 0, ctx->stream, N, in, t32a);
+  GF_TRY(gf_bsr_ilu_apply_f32(ilu, mlu32, t32a, t32b, t32c));
+  hipLaunchKernelGGL(to_f64, grid1(N), dim3(256), 0, ctx->stream, N, t32c, out);
+  return check_launch(ctx, "apply_prec mixed");
+}
+
+int Operator::apply_prec_a(const double* in, double* out, bool mixed) const {
+  const size_t N = 6 * (size_t)n;
+  if (!mixed) {
+    GF_TRY(gf_bsr_ilu_apply_f64(ilu, mlu, in, work64, work64b));
+    return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, work64b, out);
+  }
+  hipLaunchKernelGGL(to_f32, grid1(N), dim3(256), 0, ctx->stream, N, in, t32a);
+  GF_TRY(gf_bsr_ilu_apply_f32(ilu, mlu32, t32a, t32b, t32c));
+  GF_TRY(gf_bsr_spmv_f32(ctx, n, krowp, kcols, kvals32, t32c, t32a));
+  hipLaunchKernelGGL(to_f64, grid1(N), dim3(256), 0, ctx->stream, N, t32a, out);
+  return check_launch(ctx, "apply_prec_a mixed");
+}
+
+}  // namespace gf

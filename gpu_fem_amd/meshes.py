@@ -1,0 +1,209 @@
+"""Seeded mesh, boundary-condition and load generators for the linear-static
+shell workloads (host side, numpy).
+
+Reference: ``examples/plate_demo/_plate_utils.h:9-144`` (createPlateAssembler),
+``:146-184`` (getPlateLoads), ``:186-212`` (getPlatePointLoad).  The reference
+has no cylinder or wingbox generator; those are defined here.
+
+Connectivity is emitted in the basis' tensor order {(-1,-1), (1,-1), (-1,1),
+(1,1)} (``include/shell/basis.h:138-142``).  ``ccw=True`` reproduces the
+reference plate generator's counter-clockwise order {n, n+1, n+nnx+1, n+nnx}
+(``_plate_utils.h:103-104``), which the reference's own fixtures were made
+with.
+
+Every generator returns a :class:`Mesh`; element data rows are
+``(E, nu, thick, tOffset)``; BCs are DOF indices ``6*node + k`` in the
+unpermuted node numbering; loads have ``6*nnodes`` entries.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RT3 = 0.577350269189626  # include/shell/quadrature.h:9
+REF_PI = 3.1415926535897  # examples/plate_demo/_plate_utils.h:165
+
+
+@dataclass
+class Mesh:
+    conn: np.ndarray       # int32 [nelems, 4]
+    xpts: np.ndarray       # float64 [nnodes, 3]
+    data: np.ndarray       # float64 [nelems, 4]
+    bcs: np.ndarray        # int32 [nbcs]
+    loads: np.ndarray      # float64 [6*nnodes]
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nnodes(self) -> int:
+        return self.xpts.shape[0]
+
+    @property
+    def nelems(self) -> int:
+        return self.conn.shape[0]
+
+
+def _data(nelems, E, nu, thick, t_offset=0.0):
+    d = np.empty((nelems, 4))
+    d[:, 0] = E
+    d[:, 1] = nu
+    d[:, 2] = thick
+    d[:, 3] = t_offset
+    return d
+
+
+def plate_conn(nxe, nye, ccw=False):
+    nnx = nxe + 1
+    iy, ix = np.meshgrid(np.arange(nye), np.arange(nxe), indexing="ij")
+    n0 = (nnx * iy + ix).ravel()
+    if ccw:
+        conn = np.stack([n0, n0 + 1, n0 + nnx + 1, n0 + nnx], axis=1)
+    else:
+        conn = np.stack([n0, n0 + 1, n0 + nnx, n0 + nnx + 1], axis=1)
+    return conn.astype(np.int32)
+
+
+def plate_xpts(nxe, nye, Lx, Ly):
+    nnx, nny = nxe + 1, nye + 1
+    dx, dy = Lx / nxe, Ly / nye
+    iy, ix = np.meshgrid(np.arange(nny), np.arange(nnx), indexing="ij")
+    x = np.zeros((nnx * nny, 3))
+    x[:, 0] = (dx * ix).ravel()
+    x[:, 1] = (dy * iy).ravel()
+    return x
+
+
+def plate_bcs_ss_reference(nxe, nye):
+    """Simply supported pattern of ``_plate_utils.h:51-82`` (same order)."""
+    nnx, nny = nxe + 1, nye + 1
+    bcs = list(range(6))
+    for ix in range(1, nnx):
+        n = ix
+        bcs += [6 * n + 1, 6 * n + 2]
+    for iy in range(1, nny):
+        n = nnx * iy
+        bcs += [6 * n, 6 * n + 2]
+        n = nnx * iy + nnx - 1
+        bcs += [6 * n + 2]
+    for ix in range(1, nnx - 1):
+        n = nnx * (nny - 1) + ix
+        bcs += [6 * n + 2]
+    return np.array(bcs, np.int32)
+
+
+def plate_bcs_clamped(nxe, nye):
+    nnx, nny = nxe + 1, nye + 1
+    iy, ix = np.meshgrid(np.arange(nny), np.arange(nnx), indexing="ij")
+    edge = ((ix == 0) | (ix == nnx - 1) | (iy == 0) | (iy == nny - 1)).ravel()
+    nodes = np.nonzero(edge)[0]
+    return (6 * nodes[:, None] + np.arange(6)[None, :]).ravel().astype(np.int32)
+
+
+def plate_loads_nodal_reference(nxe, nye, Lx, Ly, load_mag):
+    """Point-sampled sine load of ``_plate_utils.h:146-184`` (w DOF only)."""
+    nnx, nny = nxe + 1, nye + 1
+    dx, dy = Lx / nxe, Ly / nye
+    iy, ix = np.meshgrid(np.arange(nny), np.arange(nnx), indexing="ij")
+    x, y = ix * dx, iy * dy
+    f = np.zeros(6 * nnx * nny)
+    f[2::6] = (load_mag * np.sin(REF_PI * x / Lx) *
+               np.sin(REF_PI * y / Ly)).ravel()
+    return f
+
+
+def consistent_pressure_loads(conn, xpts, pressure, normal_fn=None, order=3):
+    """Nodal forces f_a = int p(x) N_a n dA over every element.
+
+    ``pressure(x)`` gives the pressure at points x [m, 3]; the force acts
+    along the element normal (``normal_fn`` may override it).  Gauss order
+    ``order`` per direction on the bilinear map (tensor node order).
+    """
+    g, w = np.polynomial.legendre.leggauss(order)
+    X = xpts[conn]  # [ne, 4, 3]
+    f = np.zeros((xpts.shape[0], 3))
+    xa = np.array([-1.0, 1.0, -1.0, 1.0])
+    ea = np.array([-1.0, -1.0, 1.0, 1.0])
+    for i, xi in enumerate(g):
+        for j, eta in enumerate(g):
+            N = 0.25 * (1 + xa * xi) * (1 + ea * eta)
+            Nxi = 0.25 * xa * (1 + ea * eta)
+            Neta = 0.25 * ea * (1 + xa * xi)
+            x = np.einsum("a,eak->ek", N, X)
+            Xxi = np.einsum("a,eak->ek", Nxi, X)
+            Xeta = np.einsum("a,eak->ek", Neta, X)
+            nrm = np.cross(Xxi, Xeta)  # |nrm| = dA / (dxi deta)
+            if normal_fn is not None:
+                dA = np.linalg.norm(nrm, axis=1)
+                nrm = normal_fn(x) * dA[:, None]
+            p = pressure(x)
+            for a in range(4):
+                np.add.at(f, conn[:, a], (w[i] * w[j] * N[a] * p)[:, None] * nrm)
+    out = np.zeros(6 * xpts.shape[0])
+    out[0::6], out[1::6], out[2::6] = f[:, 0], f[:, 1], f[:, 2]
+    return out
+
+
+def plate(nxe, nye, Lx=1.0, Ly=1.0, E=70e9, nu=0.3, thick=0.01,
+          bc="clamped", load="uniform", load_mag=1e3, ccw=False):
+    """Flat plate in the z = 0 plane.
+
+    bc: "clamped" (all 6 DOFs on every edge node) or "ss" (the reference's
+    simply supported pattern).  load: "uniform" (consistent, magnitude
+    load_mag Pa), "sine" (consistent load_mag sin(pi x/Lx) sin(pi y/Ly)) or
+    "nodal_sine" (the reference's point-sampled nodal values).
+    """
+    conn = plate_conn(nxe, nye, ccw=ccw)
+    xpts = plate_xpts(nxe, nye, Lx, Ly)
+    bcs = (plate_bcs_clamped(nxe, nye) if bc == "clamped"
+           else plate_bcs_ss_reference(nxe, nye))
+    if load == "uniform":
+        loads = consistent_pressure_loads(conn, xpts,
+                                          lambda x: np.full(len(x), load_mag))
+    elif load == "sine":
+        loads = consistent_pressure_loads(
+            conn, xpts, lambda x: load_mag * np.sin(np.pi * x[:, 0] / Lx) *
+            np.sin(np.pi * x[:, 1] / Ly))
+    elif load == "nodal_sine":
+        loads = plate_loads_nodal_reference(nxe, nye, Lx, Ly, load_mag)
+    else:
+        raise ValueError(load)
+    loads[bcs] = 0.0
+    return Mesh(conn, xpts, _data(len(conn), E, nu, thick), bcs, loads,
+                name=f"plate{nxe}x{nye}",
+                meta=dict(nxe=nxe, nye=nye, Lx=Lx, Ly=Ly, bc=bc, load=load))
+
+
+def cylinder(nc, na, R=1.0, L=3.0, E=70e9, nu=0.3, t_range=(1e-3, 8e-3),
+             seed=0, pressure=50e3, axial_line_load=10e3):
+    """Open cylinder, nc circumferential x na axial quads (periodic in theta).
+
+    Nodes: nc around x (na + 1) along z.  The z = 0 end is clamped; internal
+    pressure acts outward (consistent loads) and a uniform axial line load
+    (N/m) acts on the z = L end ring.  Per-element thickness is uniform in
+    t_range from ``seed`` (numpy PCG64).
+    """
+    th = 2 * np.pi * np.arange(nc) / nc
+    z = L * np.arange(na + 1) / na
+    ZZ, TT = np.meshgrid(z, th, indexing="ij")
+    xpts = np.stack([R * np.cos(TT).ravel(), R * np.sin(TT).ravel(),
+                     ZZ.ravel()], axis=1)
+    ia, ic = np.meshgrid(np.arange(na), np.arange(nc), indexing="ij")
+    n00 = (nc * ia + ic).ravel()
+    n10 = (nc * ia + (ic + 1) % nc).ravel()
+    conn = np.stack([n00, n10, n00 + nc, n10 + nc], axis=1).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    thick = rng.uniform(t_range[0], t_range[1], size=len(conn))
+    data = _data(len(conn), E, nu, 1.0)
+    data[:, 2] = thick
+    bcs = (6 * np.arange(nc)[:, None] + np.arange(6)[None, :]).ravel()
+    radial = lambda x: np.stack([x[:, 0], x[:, 1], 0 * x[:, 0]], 1) / R
+    loads = consistent_pressure_loads(conn, xpts,
+                                      lambda x: np.full(len(x), pressure),
+                                      normal_fn=radial)
+    end = nc * na + np.arange(nc)
+    loads[6 * end + 2] += axial_line_load * (2 * np.pi * R / nc)
+    loads[bcs] = 0.0
+    return Mesh(conn, xpts, data, bcs.astype(np.int32), loads,
+                name=f"cylinder{nc}x{na}",
+                meta=dict(nc=nc, na=na, R=R, L=L, seed=seed))

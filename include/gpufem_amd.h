@@ -1,0 +1,200 @@
+/*
+ * gpufem_amd -- C ABI of the MI355X (gfx950) linear-static MITC4 shell path.
+ *
+ * The reference (smdogroup/gpu_fem) has no C ABI: its operator surface is a
+ * header-only C++ template library.  Each entry point below replaces the
+ * reference function or closed-library call cited beside it; the C++ host
+ * classes in include/gpufem/ (ElementAssembler, BsrData, BsrMat, ...) keep the
+ * reference call surface and are built on these functions.
+ *
+ * Conventions
+ *   - every function returns an int status (GF_OK = 0, negative on error);
+ *     gf_context_last_error() describes the last failure on a context
+ *   - index arrays are int32; BSR blocks are 6x6, row-major inside the block
+ *     (val[36*b + 6*r + c]); perm[old] = new, iperm[new] = old
+ *   - "d_" pointers are device memory (hipMalloc'ed by the caller), all other
+ *     pointers are host memory; device work is stream-ordered on the context
+ *     stream and nothing on the hot path synchronises the device
+ *   - host arrays returned through out-pointers are allocated by the library
+ *     and released with gf_free()
+ *   - a context is not thread-safe: use one context per host thread
+ */
+#ifndef GPUFEM_AMD_H
+#define GPUFEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GF_OK = 0,
+  GF_ERR_INVALID = -1,       /* bad argument or shape */
+  GF_ERR_NOMEM = -2,         /* device or host allocation failed */
+  GF_ERR_DEVICE = -3,        /* HIP runtime error */
+  GF_ERR_ZERO_PIVOT = -4,    /* singular diagonal block in ILU */
+  GF_ERR_BREAKDOWN = -5,     /* GMRES breakdown (H[j+1,j] == 0 early) */
+  GF_ERR_NO_CONVERGENCE = -6,/* max iterations reached */
+  GF_ERR_OVER_BUDGET = -7,   /* serving admission over the HBM budget */
+  GF_ERR_TIMEOUT = -8        /* a bounded device wait expired */
+};
+
+typedef struct gf_context_s* gf_context;
+typedef struct gf_ilu_s* gf_ilu;
+typedef struct gf_problem_s* gf_problem;
+
+/* ---- context (replaces the implicit default stream + exit() on error of
+ *      include/cuda_utils.h:33-83) ---- */
+int gf_context_create(int device, gf_context* out);
+int gf_context_destroy(gf_context ctx);
+int gf_context_synchronize(gf_context ctx);
+const char* gf_context_last_error(gf_context ctx);
+void* gf_context_stream(gf_context ctx);
+void gf_free(void* host_ptr);
+int gf_version(void);
+
+/* ---- host symbolic phase (include/linalg/bsr_utils.h, reordering_utils.h) */
+/* node adjacency incl. self, sorted (bsr_utils.h:419-476 get_row_col_ptrs*) */
+int gf_node_graph(int nelems, int nnodes, int nodes_per_elem, const int* conn,
+                  int** rowp, int** cols, int* nnzb);
+/* reverse Cuthill-McKee, rcm_vars[old] = new
+ * (reordering_utils.h:272 TacsComputeRCMOrder) */
+int gf_rcm_order(int nnodes, const int* rowp, const int* cols, int root,
+                 int iters, int* rcm_vars);
+/* approximate minimum degree, perm[old] = new (bsr_utils.h:510-567 via
+ * sparse-utils BSRMatAMDFactorSymbolicCUDA, which is not available) */
+int gf_amd_order(int nnodes, const int* rowp, const int* cols, int* perm);
+/* B(perm[i], perm[j]) != 0 iff A(i, j) != 0, sorted (gmres.cu:242-252
+ * BSRMatApplyPerm) */
+int gf_pattern_permute(int nnodes, const int* rowp, const int* cols,
+                       const int* perm, int** prowp, int** pcols);
+/* level-of-fill ILU(k) pattern (bsr_utils.h:678 computeILUk) */
+int gf_iluk_symbolic(int nnodes, const int* rowp, const int* cols,
+                     int lev_fill, int** frowp, int** fcols, int* nnzb);
+/* block index of (perm[conn[n]], perm[conn[m]]) per element
+ * (bsr_utils.h:809 get_elem_ind_map); map has nelems*npe*npe entries */
+int gf_elem_ind_map(int nelems, int nodes_per_elem, const int* conn,
+                    const int* rowp, const int* cols, const int* perm,
+                    int* map);
+/* transpose pattern + transpose_block_map (bsr_utils.h:157-211) */
+int gf_transpose_map(int nnodes, const int* rowp, const int* cols, int** trowp,
+                     int** tcols, int** tmap);
+/* max |i - j| over the pattern (bsr_utils.h:655 getBandWidth) */
+int gf_bandwidth(int nnodes, const int* rowp, const int* cols);
+
+/* ---- device kernels on caller-owned device buffers ---- */
+/* MITC4 Jacobian + residual (assembler.h:267 add_jacobian ->
+ * elem_group.cuh:101 add_jacobian_gpu).  data = {E, nu, thick, tOffset} per
+ * element.  kvals are accumulated into (callers zero them first), res is
+ * accumulated at 6*perm[node]+k; vars may be NULL (zero displacement), res
+ * may be NULL. */
+int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
+                                   const int* d_conn, const double* d_xpts,
+                                   const double* d_vars, const double* d_data,
+                                   const int* d_elem_ind_map,
+                                   const int* d_perm, double* d_kvals,
+                                   double* d_res);
+/* residual only (assembler.h:232 add_residual) */
+int gf_assemble_residual_mitc4_f64(gf_context ctx, int nelems, int nnodes,
+                                   const int* d_conn, const double* d_xpts,
+                                   const double* d_vars, const double* d_data,
+                                   const int* d_perm, double* d_res);
+/* strain energy (assembler.h:198 add_energy); writes one double */
+int gf_energy_mitc4_f64(gf_context ctx, int nelems, const int* d_conn,
+                        const double* d_xpts, const double* d_vars,
+                        const double* d_data, double* d_energy);
+/* zero rows and columns of constrained DOFs, unit diagonal
+ * (bsr_mat.cuh:2-104 apply_mat_bcs_rows/cols_kernel) */
+int gf_bc_apply_mat(gf_context ctx, int nbcs, const int* d_bcs,
+                    const int* d_rowp, const int* d_cols, const int* d_trowp,
+                    const int* d_tcols, const int* d_tmap, const int* d_perm,
+                    double* d_vals);
+/* vec[bc] = 0 (vec.cuh:4 apply_vec_bcs_kernel); d_perm may be NULL for an
+ * unpermuted vector */
+int gf_bc_apply_vec(gf_context ctx, int nbcs, const int* d_bcs,
+                    const int* d_perm, double* d_vec);
+/* copy K blocks into a superset pattern, zero elsewhere
+ * (bsr_mat.cuh:107 copy_mat_values_kernel) */
+int gf_bsr_copy_to_pattern(gf_context ctx, int nnodes, const int* d_rowp,
+                           const int* d_cols, const double* d_vals,
+                           const int* d_trowp, const int* d_tcols,
+                           double* d_tvals);
+/* out[6*perm[i]+k] = in[6*i+k] (vec.cuh:16 permute_vec_kernel) */
+int gf_permute_vec(gf_context ctx, int nnodes, const int* d_perm,
+                   const double* d_in, double* d_out);
+/* y = A x (cusparseDbsrmv, gmres.cu:588) */
+int gf_bsr_spmv_f64(gf_context ctx, int nnodes, const int* d_rowp,
+                    const int* d_cols, const double* d_vals, const double* d_x,
+                    double* d_y);
+int gf_bsr_spmv_f32(gf_context ctx, int nnodes, const int* d_rowp,
+                    const int* d_cols, const float* d_vals, const float* d_x,
+                    float* d_y);
+int gf_convert_f64_f32(gf_context ctx, size_t n, const double* d_in,
+                       float* d_out);
+
+/* ---- ILU analysis, numeric factorisation and triangular sweeps
+ *      (cusparseXbsrilu02 / cusparseXbsrsv2, gmres.cu:374-465, 609-625) ---- */
+/* host pattern in, uploads the pattern and the level schedules */
+int gf_ilu_create(gf_context ctx, int nnodes, const int* rowp, const int* cols,
+                  gf_ilu* out);
+int gf_ilu_destroy(gf_ilu ilu);
+int gf_ilu_info(gf_ilu ilu, int* nnzb, int* nlevels_lower,
+                int* nlevels_upper);
+const int* gf_ilu_device_rowp(gf_ilu ilu);
+const int* gf_ilu_device_cols(gf_ilu ilu);
+/* in place on device values laid out on the ILU pattern; on
+ * GF_ERR_ZERO_PIVOT *zero_pivot holds the first singular block row */
+int gf_bsr_ilu_factor_f64(gf_ilu ilu, double* d_vals, int* zero_pivot);
+int gf_bsr_ilu_factor_f32(gf_ilu ilu, float* d_vals, int* zero_pivot);
+/* x = U^-1 L^-1 b with the factored values (b and x may not alias) */
+int gf_bsr_ilu_apply_f64(gf_ilu ilu, const double* d_lu, const double* d_b,
+                         double* d_tmp, double* d_x);
+int gf_bsr_ilu_apply_f32(gf_ilu ilu, const float* d_lu, const float* d_b,
+                         float* d_tmp, float* d_x);
+
+/* ---- complete linear-static problem (examples/iterative/gmres.cu flow) ---- */
+typedef struct {
+  int reorder;       /* 0 natural, 1 RCM (root 0), 2 AMD */
+  int lev_fill;      /* ILU(k) level of fill */
+  int restart;       /* GMRES(m) restart length */
+  int max_iters;     /* total Krylov iterations */
+  double rtol;       /* ||b - A x|| <= rtol ||b|| in FP64 */
+  int precision;     /* 0 FP64, 1 mixed (FP32 ILU/trsv/SpMV, FP64 outer) */
+} gf_options;
+
+typedef struct {
+  int status;        /* GF_OK or an error code */
+  int iterations;    /* Krylov iterations */
+  int outer;         /* restart cycles (FP64) or refinement steps (mixed) */
+  double rel_resid;  /* true ||b - A x|| / ||b|| in FP64 */
+  double t_assemble, t_factor, t_solve; /* seconds (device, stream-timed) */
+} gf_solve_info;
+
+void gf_options_default(gf_options* opt);
+/* host mesh in (copied); runs the symbolic phase and uploads everything */
+int gf_problem_create(gf_context ctx, int nnodes, int nelems, const int* conn,
+                      const double* xpts, const double* data, int nbcs,
+                      const int* bcs, const gf_options* opt, gf_problem* out);
+int gf_problem_destroy(gf_problem p);
+/* K = add_jacobian (+ BCs) and copy into the preconditioner pattern */
+int gf_problem_assemble(gf_problem p);
+/* numeric ILU in the configured precision */
+int gf_problem_factor(gf_problem p);
+/* solve K x = loads; loads/x are unpermuted host or device arrays of
+ * 6*nnodes doubles (device when is_device != 0) */
+int gf_problem_solve(gf_problem p, const double* loads, double* x,
+                     int is_device, gf_solve_info* info);
+/* sizes and device pointers for tests and benchmarks */
+int gf_problem_sizes(gf_problem p, int* nnodes, int* nnzb_k, int* nnzb_m);
+int gf_problem_device_arrays(gf_problem p, const int** krowp,
+                             const int** kcols, double** kvals,
+                             const int** mrowp, const int** mcols,
+                             double** mvals, const int** perm);
+int gf_problem_host_perm(gf_problem p, int* perm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUFEM_AMD_H */

@@ -1,0 +1,146 @@
+"""End-to-end linear-static solves on the GPU through gf_problem_* against
+the oracle (same symbolic phase, assembly, ILU and GMRES restated on the CPU)
+and against a direct solve.
+
+Tolerances: index arrays bit-exact; displacement within 1e-8 relative
+(2-norm) of the oracle's FP64 solution; the true relative residual
+||b - K x|| / ||b|| evaluated by the oracle in FP64 at or below the requested
+tolerance (1e-10 for the small plate, 1e-8 otherwise) for both the FP64 and
+the mixed-precision path.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gpu_fem_amd as G
+import oracle
+from gpu_fem_amd import meshes as M
+
+from gpu_helpers import host, oracle_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def true_rel_resid(s, mesh, x):
+    b = oracle.apply_bcs_vec(mesh.bcs, mesh.loads)
+    bp = oracle.permute_vec(s["perm"], b)
+    xp = oracle.permute_vec(s["perm"], x)
+    r = bp - oracle.bsr_spmv(s["krp"], s["kcp"], s["kv"], xp)
+    return np.linalg.norm(r) / np.linalg.norm(bp)
+
+
+def oracle_solution(s, mesh, restart, rtol):
+    lu, st = oracle.bsr_ilu(s["mrp"], s["mcp"], s["mv"])
+    assert st == 0
+    b = oracle.permute_vec(s["perm"], oracle.apply_bcs_vec(mesh.bcs, mesh.loads))
+    xp, st, info = oracle.gmres(s["krp"], s["kcp"], s["kv"], s["mrp"], s["mcp"],
+                                lu, b, restart=restart, rtol=rtol,
+                                max_iters=20000)
+    assert st == 0, info
+    iperm = np.argsort(s["perm"]).astype(np.int32)
+    return oracle.permute_vec(iperm, xp), info
+
+
+CASES = [
+    # name, mesh builder, reorder id, k, restart, rtol
+    ("plate10_clamped_ilu0_natural",
+     lambda: M.plate(10, 10, 1.0, 1.0, thick=0.01, bc="clamped", load="uniform"),
+     0, 0, 50, 1e-10),
+    ("plate24_ss_rcm_ilu3",
+     lambda: M.plate(24, 24, thick=0.005, bc="ss", load="sine"), 1, 3, 50, 1e-8),
+    ("cylinder_amd_ilu2",
+     lambda: M.cylinder(24, 16, seed=11), 2, 2, 50, 1e-8),
+]
+
+
+@pytest.mark.parametrize("name,build,reorder,k,restart,rtol", CASES,
+                         ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_solve_matches_oracle(gpu_ctx, name, build, reorder, k, restart, rtol,
+                              precision):
+    say(f"[gpu] solve {name} precision={precision}")
+    mesh = build()
+    p = G.Problem(gpu_ctx, mesh, reorder=reorder, lev_fill=k, restart=restart,
+                  rtol=rtol, max_iters=20000, precision=precision)
+    perm = p.perm()
+    s = oracle_problem(mesh, {0: "natural", 1: "rcm", 2: "natural"}[reorder], k)
+    if reorder == 2:
+        # use the product's AMD permutation on the oracle side as well
+        rp, cp = oracle.node_graph(mesh.conn, mesh.nnodes)
+        s = _oracle_with_perm(mesh, rp, cp, perm, k)
+    np.testing.assert_array_equal(perm, s["perm"])
+    # index arrays bit-exact
+    _, nk, nm = p.sizes()
+    assert nk == s["krp"][-1] and nm == s["mrp"][-1]
+    p.assemble()
+    p.factor()
+    x, info = p.solve()
+    assert info.status == 0, info.status
+    x_o, info_o = oracle_solution(s, mesh, restart, rtol)
+    rel = np.linalg.norm(x - x_o) / np.linalg.norm(x_o)
+    tr = true_rel_resid(s, mesh, x)
+    say(f"    iters={info.iterations} outer={info.outer} "
+        f"rel_resid={info.rel_resid:.3e} true={tr:.3e} |x-x_oracle|={rel:.3e} "
+        f"oracle_iters={info_o['iters']}")
+    assert tr <= rtol * 1.01
+    assert rel <= 1e-8 or (rtol >= 1e-8 and rel <= 1e-6)
+    p.close()
+
+
+def _oracle_with_perm(mesh, rp, cp, perm, k):
+    krp, kcp = oracle.permute_pattern(rp, cp, perm)
+    emap = oracle.elem_ind_map(mesh.conn, krp, kcp, perm)
+    kv, _ = oracle.assemble_jacobian(mesh.conn, mesh.xpts, mesh.data, emap,
+                                     perm, int(krp[-1]), with_res=False)
+    tr, tc, tm = oracle.transpose(krp, kcp)
+    kv = oracle.apply_bcs_mat(mesh.bcs, krp, kcp, tr, tc, tm, perm, kv)
+    mrp, mcp, _ = oracle.iluk(krp, kcp, k)
+    mv = oracle.copy_values(krp, kcp, kv, mrp, mcp)
+    return dict(perm=np.asarray(perm, np.int32), krp=krp, kcp=kcp, kv=kv,
+                mrp=mrp, mcp=mcp, mv=mv, emap=emap, tr=tr, tc=tc, tm=tm)
+
+
+def test_problem_device_arrays_bit_exact(gpu_ctx):
+    say("[gpu] problem device arrays plate 9x7 rcm ilu2")
+    mesh = M.plate(9, 7, bc="ss", load="sine")
+    p = G.Problem(gpu_ctx, mesh, reorder=1, lev_fill=2)
+    s = oracle_problem(mesh, "rcm", 2)
+    import torch
+
+    d = p.device_arrays()
+    n = mesh.nnodes
+
+    p.assemble()
+    gpu_ctx.sync()
+    kvals = _copy_dev(d["kvals"], 36 * int(s["krp"][-1]), np.float64)
+    krowp = _copy_dev(d["krowp"], n + 1, np.int32)
+    kcols = _copy_dev(d["kcols"], int(s["krp"][-1]), np.int32)
+    mrowp = _copy_dev(d["mrowp"], n + 1, np.int32)
+    mcols = _copy_dev(d["mcols"], int(s["mrp"][-1]), np.int32)
+    np.testing.assert_array_equal(krowp, s["krp"])
+    np.testing.assert_array_equal(kcols, s["kcp"])
+    np.testing.assert_array_equal(mrowp, s["mrp"])
+    np.testing.assert_array_equal(mcols, s["mcp"])
+    assert np.abs(kvals - s["kv"]).max() <= 1e-12 * np.abs(s["kv"]).max()
+    p.close()
+
+
+def _copy_dev(addr, count, dtype):
+    import ctypes
+
+    import torch
+
+    out = np.empty(count, dtype=dtype)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                              ctypes.c_int]
+    torch.cuda.synchronize()
+    st = hip.hipMemcpy(out.ctypes.data, addr, out.nbytes, 2)  # D2H
+    assert st == 0
+    return out
