@@ -85,6 +85,7 @@ _SIGS = {
     "gf_ilu_info": [_P, _PI, _PI, _PI],
     "gf_ilu_device_rowp": [_P],
     "gf_ilu_device_cols": [_P],
+    "gf_ilu_check": [_P],
     "gf_bsr_ilu_factor_f64": [_P, _P, _PI],
     "gf_bsr_ilu_factor_f32": [_P, _P, _PI],
     "gf_bsr_ilu_apply_f64": [_P, _P, _P, _P, _P],
@@ -295,6 +296,7 @@ class Ilu:
         fn = (lib().gf_bsr_ilu_apply_f64 if d_lu.element_size() == 8
               else lib().gf_bsr_ilu_apply_f32)
         self.ctx.check(fn(self.h, ptr(d_lu), ptr(d_b), ptr(d_tmp), ptr(d_x)))
+        self.ctx.check(lib().gf_ilu_check(self.h))
 
     def close(self):
         if self.h:

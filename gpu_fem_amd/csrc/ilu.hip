@@ -8,10 +8,22 @@
 // diagonal, upper blocks hold U_ij.  The L sweep has a unit diagonal, the U
 // sweep a non-unit one.
 //
-// Schedule: rows are grouped by dependency level on the host (the lower
-// levels of the factorisation equal those of the L sweep).  Each level is
-// one launch; rows inside a level are independent.
+// Schedules
+//   factorisation: rows grouped by dependency level on the host (the levels
+//     of the factorisation equal those of the L sweep); one launch per level.
+//   sweeps: one persistent launch per sweep.  A wavefront takes the next row
+//     in topological order from a 64-bit counter, prefetches the row's
+//     blocks, waits on per-row ready flags of the rows it depends on
+//     (agent-scope acquire, bounded spin), reduces across lanes, solves the
+//     6x6 diagonal block with lane shuffles, stores, and publishes its flag
+//     (agent-scope release).  Rows are handed out in dependency order, so
+//     every row waited on is already held by a running wavefront and the
+//     sweep cannot deadlock; flags carry a sweep epoch, so they are never
+//     cleared.  The level-by-level sweep stays available
+//     (GPUFEM_TRSV=level) as an independent schedule for parity checks.
 #include <algorithm>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -25,6 +37,14 @@ struct gf_ilu_s {
   std::vector<int> lptr, uptr;             // level offsets (host)
   int* pivot = nullptr;                    // first zero-pivot row (device)
   int max_row_len = 0;
+  // sync-free sweep state
+  int* flags = nullptr;                    // per-row epoch of completion
+  unsigned long long* counter = nullptr;   // row dispenser (monotonic)
+  int* err = nullptr;                      // set when a bounded wait expires
+  unsigned long long base = 0;             // counter value at sweep start
+  int epoch = 0;
+  int grid = 0;                            // workgroups per sweep launch
+  bool level_sweeps = false;
 };
 
 namespace gf {
@@ -173,6 +193,93 @@ __global__ void __launch_bounds__(256)
   if (active) x[6 * (size_t)i + r] = s;
 }
 
+
+constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
+constexpr int kPrefetchBlocks = 24;       // L/U blocks held in registers
+constexpr unsigned kSpinLimit = 1u << 22; // bounded wait per dependency
+
+template <typename T, bool kLower>
+__global__ void __launch_bounds__(kSweepThreads)
+    trsv_syncfree_kernel(int n, const int* __restrict__ rowp,
+                         const int* __restrict__ cols,
+                         const int* __restrict__ diag,
+                         const T* __restrict__ lu, const T* __restrict__ b,
+                         T* x, int* flags, int epoch,
+                         unsigned long long* counter, unsigned long long base,
+                         int* err) {
+  const int lane = threadIdx.x % 64;
+  const int r = lane / 6, c = lane % 6;  // lanes 0..35: entry (r, c)
+  const bool ent = lane < 36;
+  for (;;) {
+    unsigned long long t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1ull);
+    t = __shfl(t, 0, 64);
+    const long long k = (long long)(t - base);
+    if (k >= n) break;
+    const int i = kLower ? (int)k : n - 1 - (int)k;
+    const int di = diag[i];
+    const int k0 = kLower ? rowp[i] : di + 1;
+    const int k1 = kLower ? di : rowp[i + 1];
+    const int nb = k1 - k0;
+    // prefetch the row's off-diagonal blocks, its diagonal-block row and rhs
+    T pre[kPrefetchBlocks];
+#pragma unroll
+    for (int q = 0; q < kPrefetchBlocks; q++)
+      pre[q] = (ent && q < nb) ? lu[36 * (size_t)(k0 + q) + lane] : T(0);
+    T drow[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+      drow[m] = (lane < 6) ? lu[36 * (size_t)di + 6 * lane + m] : T(0);
+    T s = (lane < 6) ? b[6 * (size_t)i + lane] : T(0);
+    // wait for the rows this one depends on
+    for (int q = lane; q < nb; q += 64) {
+      const int j = cols[k0 + q];
+      unsigned spins = 0;
+      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // partial products, lane (r, c): sum_q A[q][r][c] x[col_q][c]
+    T acc = 0;
+#pragma unroll
+    for (int q = 0; q < kPrefetchBlocks; q++)
+      if (ent && q < nb) acc += pre[q] * x[6 * (size_t)cols[k0 + q] + c];
+    for (int q = kPrefetchBlocks; q < nb; q++)
+      if (ent) acc += lu[36 * (size_t)(k0 + q) + lane] * x[6 * (size_t)cols[k0 + q] + c];
+    // reduce over c into lane r (lanes 0..5 then hold row sums)
+    T rs = 0;
+#pragma unroll
+    for (int m = 0; m < 6; m++) rs += __shfl(acc, 6 * (lane % 6) + m, 64);
+    s -= (lane < 6) ? rs : T(0);
+    // 6x6 diagonal block: unit lower (L) or upper with diagonal (U)
+    if (kLower) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        const T ym = __shfl(s, m, 64);
+        if (lane < 6 && lane > m) s -= drow[m] * ym;
+      }
+    } else {
+#pragma unroll
+      for (int m = 5; m >= 0; m--) {
+        if (lane == m) s /= drow[m];
+        const T ym = __shfl(s, m, 64);
+        if (lane < 6 && lane < m) s -= drow[m] * ym;
+      }
+    }
+    if (lane < 6) x[6 * (size_t)i + lane] = s;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0)
+      __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 }  // namespace
 }  // namespace gf
 
@@ -219,9 +326,26 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
       (st = up(&h->diag, diag.data(), n)) ||
       (st = up(&h->lrows, lrows.data(), n)) ||
       (st = up(&h->urows, urows.data(), n)) ||
-      (st = gf::check_hip(ctx, hipMalloc(&h->pivot, sizeof(int)), "malloc"))) {
+      (st = gf::check_hip(ctx, hipMalloc(&h->pivot, sizeof(int)), "malloc")) ||
+      (st = gf::check_hip(ctx, hipMalloc(&h->flags, sizeof(int) * n), "malloc")) ||
+      (st = gf::check_hip(ctx, hipMalloc(&h->counter, sizeof(unsigned long long)),
+                          "malloc")) ||
+      (st = gf::check_hip(ctx, hipMalloc(&h->err, sizeof(int)), "malloc")) ||
+      (st = gf::check_hip(ctx, hipMemsetAsync(h->flags, 0, sizeof(int) * n, ctx->stream),
+                          "memset")) ||
+      (st = gf::check_hip(ctx, hipMemsetAsync(h->counter, 0, sizeof(unsigned long long),
+                                              ctx->stream), "memset")) ||
+      (st = gf::check_hip(ctx, hipMemsetAsync(h->err, 0, sizeof(int), ctx->stream),
+                          "memset"))) {
     gf_ilu_destroy(h);
     return st;
+  }
+  {
+    // enough resident wavefronts to cover the device, capped by the rows
+    const int waves = std::max(1, std::min(n, 16 * std::max(ctx->num_cus, 1)));
+    h->grid = (waves + 3) / 4;
+    const char* env = getenv("GPUFEM_TRSV");
+    h->level_sweeps = env && std::string(env) == "level";
   }
   st = gf::check_hip(ctx, hipStreamSynchronize(ctx->stream), "ilu upload");
   if (st) {
@@ -240,6 +364,9 @@ int gf_ilu_destroy(gf_ilu h) {
   hipFree(h->lrows);
   hipFree(h->urows);
   hipFree(h->pivot);
+  hipFree(h->flags);
+  hipFree(h->counter);
+  hipFree(h->err);
   delete h;
   return GF_OK;
 }
@@ -252,6 +379,21 @@ int gf_ilu_info(gf_ilu h, int* nnzb, int* nl, int* nu) {
   return GF_OK;
 }
 const int* gf_ilu_device_rowp(gf_ilu h) { return h ? h->rowp : nullptr; }
+
+int gf_ilu_check(gf_ilu h) {
+  if (!h) return GF_ERR_INVALID;
+  int e = 0;
+  GF_HIP(h->ctx, hipMemcpyAsync(&e, h->err, sizeof(int), hipMemcpyDeviceToHost,
+                                h->ctx->stream));
+  GF_HIP(h->ctx, hipStreamSynchronize(h->ctx->stream));
+  if (e) {
+    const int zero = 0;
+    hipMemcpy(h->err, &zero, sizeof(int), hipMemcpyHostToDevice);
+    return gf::set_error(h->ctx, GF_ERR_TIMEOUT,
+                         "triangular sweep: a dependency wait expired");
+  }
+  return GF_OK;
+}
 const int* gf_ilu_device_cols(gf_ilu h) { return h ? h->cols : nullptr; }
 
 }  // extern "C"
@@ -288,6 +430,20 @@ int ilu_apply(gf_ilu h, const T* lu, const T* b, T* tmp, T* x) {
   if (!h || !lu || !b || !tmp || !x || b == tmp || tmp == x)
     return GF_ERR_INVALID;
   gf_context ctx = h->ctx;
+  if (!h->level_sweeps) {
+    const unsigned long long waves = 4ull * h->grid;
+    hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true>), dim3(h->grid),
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
+                       h->cols, h->diag, lu, b, tmp, h->flags, ++h->epoch,
+                       h->counter, h->base, h->err);
+    h->base += (unsigned long long)h->n + waves;
+    hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false>), dim3(h->grid),
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
+                       h->cols, h->diag, lu, tmp, x, h->flags, ++h->epoch,
+                       h->counter, h->base, h->err);
+    h->base += (unsigned long long)h->n + waves;
+    return gf::check_launch(ctx, "trsv_syncfree_kernel");
+  }
   const int nl = (int)h->lptr.size() - 1;
   for (int l = 0; l < nl; l++) {
     const int cnt = h->lptr[l + 1] - h->lptr[l];

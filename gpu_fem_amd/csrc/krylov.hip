@@ -320,6 +320,10 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     hipLaunchKernelGGL(axpy_k, grid1(N), dim3(256), 0, s, N, 1.0, W.z, x);
     GF_TRY(check_launch(ctx, "gmres update"));
   }
+  if (op.ilu) {
+    const int ck = gf_ilu_check(op.ilu);
+    if (ck != GF_OK) status = ck;
+  }
   if (res) {
     res->iterations = it;
     res->outer = outer;

@@ -144,6 +144,9 @@ int gf_ilu_info(gf_ilu ilu, int* nnzb, int* nlevels_lower,
                 int* nlevels_upper);
 const int* gf_ilu_device_rowp(gf_ilu ilu);
 const int* gf_ilu_device_cols(gf_ilu ilu);
+/* GF_ERR_TIMEOUT if a sweep's bounded dependency wait expired since the last
+ * check (synchronises the context stream) */
+int gf_ilu_check(gf_ilu ilu);
 /* in place on device values laid out on the ILU pattern; on
  * GF_ERR_ZERO_PIVOT *zero_pivot holds the first singular block row */
 int gf_bsr_ilu_factor_f64(gf_ilu ilu, double* d_vals, int* zero_pivot);

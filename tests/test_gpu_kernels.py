@@ -237,3 +237,35 @@ def test_zero_pivot_reported(gpu_ctx):
         ilu.factor(dev(mv))
     assert e.value.code == -4
     ilu.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_syncfree_sweeps_match_level_schedule(gpu_ctx, dtype, monkeypatch):
+    """The persistent sync-free sweeps and the level-by-level sweeps are two
+    schedules of the same arithmetic: results agree to the last bits
+    (identical operation order per row)."""
+    import torch
+
+    say(f"[gpu] sweep schedules dtype={np.dtype(dtype).name} plate 30x30 rcm ilu3")
+    m = M.plate(30, 30, bc="ss", load="sine")
+    s = oracle_problem(m, "rcm", 3)
+    lu, _ = oracle.bsr_ilu(s["mrp"], s["mcp"], s["mv"])
+    d_lu = dev(lu.astype(dtype))
+    b = np.random.default_rng(4).standard_normal(6 * m.nnodes).astype(dtype)
+    outs = []
+    for mode in ("level", "syncfree"):
+        monkeypatch.setenv("GPUFEM_TRSV", mode)
+        ilu = G.Ilu(gpu_ctx, s["mrp"], s["mcp"])
+        d_b = dev(b)
+        d_t, d_x = torch.zeros_like(d_b), torch.zeros_like(d_b)
+        for rep in range(3):  # repeated sweeps reuse flags via epochs
+            ilu.apply(d_lu, d_b, d_t, d_x)
+        gpu_ctx.sync()
+        outs.append(host(d_x))
+        ilu.close()
+    scale = np.abs(outs[0]).max()
+    tol = 1e-13 if dtype == np.float64 else 1e-5
+    assert np.abs(outs[0] - outs[1]).max() <= tol * scale
+    x_o = oracle.bsr_trsv(s["mrp"], s["mcp"], lu, b.astype(np.float64))
+    if dtype == np.float64:
+        assert np.abs(outs[1] - x_o).max() <= 1e-10 * np.abs(x_o).max()
