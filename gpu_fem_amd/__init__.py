@@ -90,6 +90,8 @@ _SIGS = {
     "gf_bsr_ilu_factor_f32": [_P, _P, _PI],
     "gf_bsr_ilu_apply_f64": [_P, _P, _P, _P, _P],
     "gf_bsr_ilu_apply_f32": [_P, _P, _P, _P, _P],
+    "gf_gmres_solve": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I,
+                       ctypes.c_double, _I, _PI, ctypes.POINTER(ctypes.c_double)],
     "gf_options_default": [ctypes.POINTER(Options)],
     "gf_problem_create": [_P, _I, _I, _P, _P, _P, _I, _P,
                           ctypes.POINTER(Options), ctypes.POINTER(_P)],

@@ -62,9 +62,27 @@ def build(force: bool = False, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_apps()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+APPS = ["linear_static"]
+BIN = os.path.join(HERE, "bin")
+
+
+def _build_apps():
+    """C++ programs written against include/gpufem/gpufem.hpp."""
+    os.makedirs(BIN, exist_ok=True)
+    for app in APPS:
+        src = os.path.join(HERE, "apps", app + ".cpp")
+        out = os.path.join(BIN, app)
+        cmd = [HIPCC, "-O2", "-std=c++17", "-o", out, src, "-L" + HERE,
+               "-lgpufem_amd", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"build of {app} failed:\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

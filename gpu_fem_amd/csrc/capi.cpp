@@ -356,3 +356,58 @@ int gf_problem_host_perm(gf_problem p, int* perm) {
 }
 
 }  // extern "C"
+
+extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
+                              const int* kcols, const double* kvals,
+                              const float* kvals32, gf_ilu ilu,
+                              const double* mlu, const float* mlu32,
+                              const double* b, double* x, int restart,
+                              int max_iters, double rtol, int precision,
+                              int* iterations, double* rel_resid) {
+  const bool mixed = precision == 1;
+  if (!ctx || nnodes <= 0 || !krowp || !kcols || !kvals || !ilu || !b || !x ||
+      restart < 1 || restart > 512 || precision < 0 || precision > 1 ||
+      (!mixed && !mlu) || (mixed && (!mlu32 || !kvals32)))
+    return gf::set_error(ctx, GF_ERR_INVALID, "gmres_solve: bad args");
+  const size_t N = 6 * (size_t)nnodes;
+  gf::KrylovWork kw;
+  double *w64 = nullptr, *w64b = nullptr;
+  float *a = nullptr, *bb = nullptr, *c = nullptr;
+  int st = kw.init(ctx, N, restart, mixed);
+  if (!st) st = gf::check_hip(ctx, hipMalloc(&w64, sizeof(double) * N), "malloc");
+  if (!st) st = gf::check_hip(ctx, hipMalloc(&w64b, sizeof(double) * N), "malloc");
+  if (!st && mixed) {
+    st = gf::check_hip(ctx, hipMalloc(&a, sizeof(float) * N), "malloc");
+    if (!st) st = gf::check_hip(ctx, hipMalloc(&bb, sizeof(float) * N), "malloc");
+    if (!st) st = gf::check_hip(ctx, hipMalloc(&c, sizeof(float) * N), "malloc");
+  }
+  gf::GmresResult gr;
+  if (!st) {
+    gf::Operator op;
+    op.ctx = ctx;
+    op.n = nnodes;
+    op.krowp = krowp;
+    op.kcols = kcols;
+    op.kvals = kvals;
+    op.kvals32 = kvals32;
+    op.ilu = ilu;
+    op.mlu = mlu;
+    op.mlu32 = mlu32;
+    op.work64 = w64;
+    op.work64b = w64b;
+    op.t32a = a;
+    op.t32b = bb;
+    op.t32c = c;
+    st = gf::gmres_solve(kw, op, b, x, max_iters, rtol, mixed, &gr);
+  }
+  hipStreamSynchronize(ctx->stream);
+  hipFree(w64);
+  hipFree(w64b);
+  hipFree(a);
+  hipFree(bb);
+  hipFree(c);
+  kw.release();
+  if (iterations) *iterations = gr.iterations;
+  if (rel_resid) *rel_resid = gr.rel_resid;
+  return st;
+}

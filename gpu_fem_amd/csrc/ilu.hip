@@ -39,6 +39,8 @@ struct gf_ilu_s {
   int max_row_len = 0;
   // sync-free sweep state
   int* flags = nullptr;                    // per-row epoch of completion
+  int *lchain = nullptr, *uchain = nullptr; // chain offsets in sweep order
+  int nlchain = 0, nuchain = 0;
   unsigned long long* counter = nullptr;   // row dispenser (monotonic)
   int* err = nullptr;                      // set when a bounded wait expires
   unsigned long long base = 0;             // counter value at sweep start
@@ -198,86 +200,117 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 constexpr int kPrefetchBlocks = 24;       // L/U blocks held in registers
 constexpr unsigned kSpinLimit = 1u << 22; // bounded wait per dependency
 
+constexpr int kChainCache = 4;           // recent rows of a chain in registers
+
+// Rows are handed out as chains: maximal runs of consecutive rows (in sweep
+// order) where each row depends on the one before it.  A wavefront solves its
+// chain row after row; the dependencies on the last kChainCache rows of its
+// own chain come from registers, every other dependency through the flags.
+// Chains are dispensed in sweep order and every dependency of a row lies in
+// its own chain or in an earlier one, so no wavefront waits on a chain that
+// has not been dispensed.
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(kSweepThreads)
     trsv_syncfree_kernel(int n, const int* __restrict__ rowp,
                          const int* __restrict__ cols,
                          const int* __restrict__ diag,
+                         const int* __restrict__ chain_ptr, int nchains,
                          const T* __restrict__ lu, const T* __restrict__ b,
                          T* x, int* flags, int epoch,
                          unsigned long long* counter, unsigned long long base,
                          int* err) {
   const int lane = threadIdx.x % 64;
-  const int r = lane / 6, c = lane % 6;  // lanes 0..35: entry (r, c)
+  const int c = lane % 6;  // lanes 0..35: block entry (lane / 6, lane % 6)
   const bool ent = lane < 36;
   for (;;) {
     unsigned long long t = 0;
     if (lane == 0) t = atomicAdd(counter, 1ull);
     t = __shfl(t, 0, 64);
-    const long long k = (long long)(t - base);
-    if (k >= n) break;
-    const int i = kLower ? (int)k : n - 1 - (int)k;
-    const int di = diag[i];
-    const int k0 = kLower ? rowp[i] : di + 1;
-    const int k1 = kLower ? di : rowp[i + 1];
-    const int nb = k1 - k0;
-    // prefetch the row's off-diagonal blocks, its diagonal-block row and rhs
-    T pre[kPrefetchBlocks];
+    const long long kc = (long long)(t - base);
+    if (kc >= nchains) break;
+    const int c0 = chain_ptr[kc], c1 = chain_ptr[kc + 1];
+    T xc[kChainCache];  // x[row - d][c] for d = 1..kChainCache
 #pragma unroll
-    for (int q = 0; q < kPrefetchBlocks; q++)
-      pre[q] = (ent && q < nb) ? lu[36 * (size_t)(k0 + q) + lane] : T(0);
-    T drow[6];
+    for (int d = 0; d < kChainCache; d++) xc[d] = T(0);
+    for (int tpos = c0; tpos < c1; tpos++) {
+      const int i = kLower ? tpos : n - 1 - tpos;
+      const int done = tpos - c0;  // rows of this chain already solved
+      const int di = diag[i];
+      const int k0 = kLower ? rowp[i] : di + 1;
+      const int k1 = kLower ? di : rowp[i + 1];
+      const int nb = k1 - k0;
+      T pre[kPrefetchBlocks];
 #pragma unroll
-    for (int m = 0; m < 6; m++)
-      drow[m] = (lane < 6) ? lu[36 * (size_t)di + 6 * lane + m] : T(0);
-    T s = (lane < 6) ? b[6 * (size_t)i + lane] : T(0);
-    // wait for the rows this one depends on
-    for (int q = lane; q < nb; q += 64) {
-      const int j = cols[k0 + q];
-      unsigned spins = 0;
-      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > kSpinLimit) {
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-          break;
+      for (int q = 0; q < kPrefetchBlocks; q++)
+        pre[q] = (ent && q < nb) ? lu[36 * (size_t)(k0 + q) + lane] : T(0);
+      T drow[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++)
+        drow[m] = (lane < 6) ? lu[36 * (size_t)di + 6 * lane + m] : T(0);
+      T s = (lane < 6) ? b[6 * (size_t)i + lane] : T(0);
+      // wait for dependencies outside the register cache
+      bool ext = false;
+      for (int q = lane; q < nb; q += 64) {
+        const int j = cols[k0 + q];
+        const int dist = kLower ? i - j : j - i;
+        if (dist <= done && dist <= kChainCache) continue;
+        ext = true;
+        unsigned spins = 0;
+        while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > kSpinLimit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
         }
       }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // partial products, lane (r, c): sum_q A[q][r][c] x[col_q][c]
-    T acc = 0;
+      if (__any(ext)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // partial products, lane (r, c): sum_q A[q][r][c] x[col_q][c]
+      T acc = 0;
+      auto xdep = [&](int q) -> T {
+        const int j = cols[k0 + q];
+        const int dist = kLower ? i - j : j - i;
+        if (dist <= done && dist <= kChainCache)
+          return dist == 1 ? xc[0] : dist == 2 ? xc[1] : dist == 3 ? xc[2] : xc[3];
+        return ent ? x[6 * (size_t)j + c] : T(0);
+      };
 #pragma unroll
-    for (int q = 0; q < kPrefetchBlocks; q++)
-      if (ent && q < nb) acc += pre[q] * x[6 * (size_t)cols[k0 + q] + c];
-    for (int q = kPrefetchBlocks; q < nb; q++)
-      if (ent) acc += lu[36 * (size_t)(k0 + q) + lane] * x[6 * (size_t)cols[k0 + q] + c];
-    // reduce over c into lane r (lanes 0..5 then hold row sums)
-    T rs = 0;
+      for (int q = 0; q < kPrefetchBlocks; q++)
+        if (q < nb) acc += pre[q] * xdep(q);
+      for (int q = kPrefetchBlocks; q < nb; q++)
+        acc += (ent ? lu[36 * (size_t)(k0 + q) + lane] : T(0)) * xdep(q);
+      // reduce over c into lane r (lanes 0..5 then hold row sums)
+      T rs = 0;
 #pragma unroll
-    for (int m = 0; m < 6; m++) rs += __shfl(acc, 6 * (lane % 6) + m, 64);
-    s -= (lane < 6) ? rs : T(0);
-    // 6x6 diagonal block: unit lower (L) or upper with diagonal (U)
-    if (kLower) {
+      for (int m = 0; m < 6; m++) rs += __shfl(acc, 6 * (lane % 6) + m, 64);
+      s -= (lane < 6) ? rs : T(0);
+      // 6x6 diagonal block: unit lower (L) or upper with diagonal (U)
+      if (kLower) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) {
-        const T ym = __shfl(s, m, 64);
-        if (lane < 6 && lane > m) s -= drow[m] * ym;
+        for (int m = 0; m < 5; m++) {
+          const T ym = __shfl(s, m, 64);
+          if (lane < 6 && lane > m) s -= drow[m] * ym;
+        }
+      } else {
+#pragma unroll
+        for (int m = 5; m >= 0; m--) {
+          if (lane == m) s /= drow[m];
+          const T ym = __shfl(s, m, 64);
+          if (lane < 6 && lane < m) s -= drow[m] * ym;
+        }
       }
-    } else {
+      if (lane < 6) x[6 * (size_t)i + lane] = s;
+      // shift the register cache: xc[0] = x[i][c]
 #pragma unroll
-      for (int m = 5; m >= 0; m--) {
-        if (lane == m) s /= drow[m];
-        const T ym = __shfl(s, m, 64);
-        if (lane < 6 && lane < m) s -= drow[m] * ym;
-      }
+      for (int d = kChainCache - 1; d > 0; d--) xc[d] = xc[d - 1];
+      xc[0] = __shfl(s, c, 64);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (lane == 0)
+        __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (lane < 6) x[6 * (size_t)i + lane] = s;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0)
-      __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 }  // namespace
@@ -315,6 +348,24 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
   std::vector<int> lrows, urows;
   gf::level_schedule(n, rowp, cols, true, h->lptr, lrows);
   gf::level_schedule(n, rowp, cols, false, h->uptr, urows);
+  // chains: a row continues the previous row's chain (in sweep order) when it
+  // depends on that row
+  std::vector<int> lch{0}, uch{0};
+  for (int i = 1; i < n; i++) {
+    bool dep = false;
+    for (int k = rowp[i]; k < diag[i]; k++) dep |= cols[k] == i - 1;
+    if (!dep) lch.push_back(i);
+  }
+  lch.push_back(n);
+  for (int t = 1; t < n; t++) {
+    const int i = n - 1 - t;
+    bool dep = false;
+    for (int k = diag[i] + 1; k < rowp[i + 1]; k++) dep |= cols[k] == i + 1;
+    if (!dep) uch.push_back(t);
+  }
+  uch.push_back(n);
+  h->nlchain = (int)lch.size() - 1;
+  h->nuchain = (int)uch.size() - 1;
   auto up = [&](int** dst, const int* src, size_t cnt) -> int {
     GF_HIP(ctx, hipMalloc(dst, sizeof(int) * (cnt ? cnt : 1)));
     GF_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(int) * cnt,
@@ -326,6 +377,8 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
       (st = up(&h->diag, diag.data(), n)) ||
       (st = up(&h->lrows, lrows.data(), n)) ||
       (st = up(&h->urows, urows.data(), n)) ||
+      (st = up(&h->lchain, lch.data(), lch.size())) ||
+      (st = up(&h->uchain, uch.data(), uch.size())) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->pivot, sizeof(int)), "malloc")) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->flags, sizeof(int) * n), "malloc")) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->counter, sizeof(unsigned long long)),
@@ -341,8 +394,9 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
     return st;
   }
   {
-    // enough resident wavefronts to cover the device, capped by the rows
-    const int waves = std::max(1, std::min(n, 16 * std::max(ctx->num_cus, 1)));
+    // enough resident wavefronts to cover the device, capped by the chains
+    const int waves = std::max(1, std::min(std::max(h->nlchain, h->nuchain),
+                                           16 * std::max(ctx->num_cus, 1)));
     h->grid = (waves + 3) / 4;
     const char* env = getenv("GPUFEM_TRSV");
     h->level_sweeps = env && std::string(env) == "level";
@@ -365,6 +419,8 @@ int gf_ilu_destroy(gf_ilu h) {
   hipFree(h->urows);
   hipFree(h->pivot);
   hipFree(h->flags);
+  hipFree(h->lchain);
+  hipFree(h->uchain);
   hipFree(h->counter);
   hipFree(h->err);
   delete h;
@@ -373,6 +429,10 @@ int gf_ilu_destroy(gf_ilu h) {
 
 int gf_ilu_info(gf_ilu h, int* nnzb, int* nl, int* nu) {
   if (!h) return GF_ERR_INVALID;
+  if (getenv("GPUFEM_ILU_VERBOSE"))
+    fprintf(stderr, "ilu: n=%d nnzb=%d levels L=%d U=%d chains L=%d U=%d\n",
+            h->n, h->nnzb, (int)h->lptr.size() - 1, (int)h->uptr.size() - 1,
+            h->nlchain, h->nuchain);
   if (nnzb) *nnzb = h->nnzb;
   if (nl) *nl = (int)h->lptr.size() - 1;
   if (nu) *nu = (int)h->uptr.size() - 1;
@@ -434,14 +494,14 @@ int ilu_apply(gf_ilu h, const T* lu, const T* b, T* tmp, T* x) {
     const unsigned long long waves = 4ull * h->grid;
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true>), dim3(h->grid),
                        dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
-                       h->cols, h->diag, lu, b, tmp, h->flags, ++h->epoch,
-                       h->counter, h->base, h->err);
-    h->base += (unsigned long long)h->n + waves;
+                       h->cols, h->diag, h->lchain, h->nlchain, lu, b, tmp,
+                       h->flags, ++h->epoch, h->counter, h->base, h->err);
+    h->base += (unsigned long long)h->nlchain + waves;
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false>), dim3(h->grid),
                        dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
-                       h->cols, h->diag, lu, tmp, x, h->flags, ++h->epoch,
-                       h->counter, h->base, h->err);
-    h->base += (unsigned long long)h->n + waves;
+                       h->cols, h->diag, h->uchain, h->nuchain, lu, tmp, x,
+                       h->flags, ++h->epoch, h->counter, h->base, h->err);
+    h->base += (unsigned long long)h->nuchain + waves;
     return gf::check_launch(ctx, "trsv_syncfree_kernel");
   }
   const int nl = (int)h->lptr.size() - 1;

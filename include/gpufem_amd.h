@@ -157,6 +157,19 @@ int gf_bsr_ilu_apply_f64(gf_ilu ilu, const double* d_lu, const double* d_b,
 int gf_bsr_ilu_apply_f32(gf_ilu ilu, const float* d_lu, const float* d_b,
                          float* d_tmp, float* d_x);
 
+/* ---- right-preconditioned restarted GMRES on device BSR operands
+ *      (examples/iterative/gmres.cu:472-771 with restarts and a relative
+ *      tolerance).  precision 0: FP64 operator and ILU factors (d_mlu);
+ *      precision 1: GMRES-IR, inner FP32 ILU factors (d_mlu32) and FP32 copy
+ *      of the operator (d_kvals32), FP64 outer residual and Arnoldi.  x holds
+ *      the initial guess on entry.  info may be NULL. */
+int gf_gmres_solve(gf_context ctx, int nnodes, const int* d_krowp,
+                   const int* d_kcols, const double* d_kvals,
+                   const float* d_kvals32, gf_ilu ilu, const double* d_mlu,
+                   const float* d_mlu32, const double* d_b, double* d_x,
+                   int restart, int max_iters, double rtol, int precision,
+                   int* iterations, double* rel_resid);
+
 /* ---- complete linear-static problem (examples/iterative/gmres.cu flow) ---- */
 typedef struct {
   int reorder;       /* 0 natural, 1 RCM (root 0), 2 AMD */
