@@ -1,0 +1,65 @@
+"""The C++ host classes (include/gpufem/gpufem.hpp) drive the reference's
+examples/iterative/gmres.cu flow in gpu_fem_amd/apps/linear_static.cpp;
+its solution is checked against the oracle on the same plate
+(Lx = 2, Ly = 1, t = 5 mm, the reference's simply supported pattern and
+point-sampled sine load), and the direct path against a dense solve.
+Tolerance: 1e-8 relative (2-norm)."""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from gpu_fem_amd import meshes as M
+
+from gpu_helpers import oracle_problem
+
+pytestmark = pytest.mark.gpu
+APP = os.path.join(os.path.dirname(os.path.dirname(__file__)), "gpu_fem_amd",
+                   "bin", "linear_static")
+
+
+def run_app(*args):
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "x.bin")
+        r = subprocess.run([APP, *map(str, args[:5]), out, *args[5:]],
+                           capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stderr
+        return json.loads(r.stdout.strip().splitlines()[-1]), np.fromfile(out)
+
+
+def mesh(nx, ccw=False):
+    return M.plate(nx, nx, 2.0, 1.0, E=70e9, nu=0.3, thick=0.005, bc="ss",
+                   load="nodal_sine", load_mag=1.0, ccw=ccw)
+
+
+@pytest.mark.parametrize("rcm,k", [(0, 0), (1, 2)])
+def test_gmres_flow_matches_oracle(rcm, k):
+    print(f"[gpu] linear_static app nx=16 rcm={rcm} k={k}", flush=True)
+    info, x = run_app(16, k, rcm, 100, 1e-11)
+    m = mesh(16)
+    s = oracle_problem(m, "rcm" if rcm else "natural", k)
+    lu, _ = oracle.bsr_ilu(s["mrp"], s["mcp"], s["mv"])
+    b = oracle.permute_vec(s["perm"], oracle.apply_bcs_vec(m.bcs, m.loads))
+    xp, st, _ = oracle.gmres(s["krp"], s["kcp"], s["kv"], s["mrp"], s["mcp"],
+                             lu, b, restart=100, rtol=1e-11, max_iters=5000)
+    assert st == 0
+    xo = oracle.permute_vec(np.argsort(s["perm"]).astype(np.int32), xp)
+    assert info["nnzb_K"] == s["krp"][-1] and info["nnzb_ILU"] == s["mrp"][-1]
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_direct_path_matches_dense():
+    print("[gpu] linear_static app --direct nx=12", flush=True)
+    info, x = run_app(12, 0, 0, 10, 1e-10, "--direct")
+    m = mesh(12)
+    s = oracle_problem(m, "natural", 0)
+    from test_oracle_numeric import bsr_to_dense
+    A = bsr_to_dense(s["krp"], s["kcp"], s["kv"])
+    b = oracle.apply_bcs_vec(m.bcs, m.loads)
+    xd = oracle.dense_solve(A, b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
