@@ -102,6 +102,15 @@ _SIGS = {
     "gf_problem_sizes": [_P, _PI, _PI, _PI],
     "gf_problem_device_arrays": [_P] + [ctypes.POINTER(_P)] * 7,
     "gf_problem_host_perm": [_P, _P],
+    "gf_server_create": [_I, ctypes.c_double, _I, ctypes.POINTER(_P)],
+    "gf_server_submit": [_P, ctypes.c_longlong, _I, _I, _P, _P, _P, _I, _P,
+                         _P, ctypes.POINTER(Options)],
+    "gf_server_poll": [_P, ctypes.POINTER(ctypes.c_longlong), _P,
+                       ctypes.c_size_t, ctypes.POINTER(SolveInfo),
+                       ctypes.POINTER(ctypes.c_double)],
+    "gf_server_stats": [_P, _P],
+    "gf_server_drain": [_P],
+    "gf_server_destroy": [_P],
 }
 _RESTYPES = {"gf_context_last_error": ctypes.c_char_p,
              "gf_context_stream": _P, "gf_free": None,
@@ -381,6 +390,69 @@ class Problem:
     def close(self):
         if self.h:
             lib().gf_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop (gf_server): FIFO admission under an HBM budget, worker
+    slots with their own streams."""
+
+    STATS = ["completed", "failed", "queued", "active", "p50_s", "p99_s",
+             "peak_bytes", "elapsed_s"]
+
+    def __init__(self, budget_gb=200.0, slots=4, device=0):
+        h = _P()
+        check(lib().gf_server_create(device, budget_gb, slots, ctypes.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def submit(self, rid, mesh, loads=None, **opts):
+        o = default_options(**opts)
+        conn = _i32(mesh.conn).ravel()
+        xpts = np.ascontiguousarray(mesh.xpts, np.float64).ravel()
+        data = np.ascontiguousarray(mesh.data, np.float64).ravel()
+        bcs = _i32(mesh.bcs)
+        f = np.ascontiguousarray(mesh.loads if loads is None else loads,
+                                 np.float64)
+        self.sizes[rid] = mesh.nnodes
+        return lib().gf_server_submit(self.h, rid, mesh.nnodes, mesh.nelems,
+                                      ptr(conn), ptr(xpts), ptr(data),
+                                      len(bcs), ptr(bcs), ptr(f),
+                                      ctypes.byref(o))
+
+    def poll(self):
+        """(id, x, info, latency_s) of one finished request, or None."""
+        rid = ctypes.c_longlong()
+        info = SolveInfo()
+        lat = ctypes.c_double()
+        # size the output from the oldest finished id: poll into a buffer large
+        # enough for any submitted request
+        nmax = 6 * max(self.sizes.values()) if self.sizes else 0
+        x = np.zeros(nmax)
+        got = lib().gf_server_poll(self.h, ctypes.byref(rid), ptr(x), nmax,
+                                   ctypes.byref(info), ctypes.byref(lat))
+        if got != 1:
+            return None
+        n = 6 * self.sizes[rid.value]
+        return rid.value, x[:n].copy(), info, lat.value
+
+    def stats(self):
+        out = np.zeros(8)
+        check(lib().gf_server_stats(self.h, ptr(out)))
+        return dict(zip(self.STATS, out.tolist()))
+
+    def drain(self):
+        check(lib().gf_server_drain(self.h))
+
+    def close(self):
+        if self.h:
+            lib().gf_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

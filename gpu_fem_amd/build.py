@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build", "gpufem_amd")
 LIB = os.path.join(HERE, "libgpufem_amd.so")
-SOURCES = ["symbolic.cpp", "amd.cpp", "capi.cpp", "assembly.hip", "bsr.hip",
+SOURCES = ["symbolic.cpp", "amd.cpp", "capi.cpp", "server.cpp", "assembly.hip", "bsr.hip",
            "ilu.hip", "krylov.hip"]
 HEADERS = ["common.h", "symbolic.h", "krylov.h", "mitc4.h"]
 ARCH = os.environ.get("GPUFEM_ARCH", "gfx950")
@@ -57,7 +57,8 @@ def build(force: bool = False, verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread",
+           "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -1,0 +1,245 @@
+// Serving loop for linear-static requests (continuous admission under an HBM
+// budget, overlapping solves on the device).
+//
+// The reference has no serving component; this is the request loop of the
+// served workload: requests of differing meshes arrive over time, each is
+// admitted when a solver slot is free and its device footprint fits the
+// budget, and every admitted request runs the whole device path (symbolic
+// phase on the worker's host thread, upload, assembly, factorisation,
+// GMRES / GMRES-IR) on its own HIP stream, so the device work of concurrent
+// requests overlaps.  Results are collected with gf_server_poll.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  long long id = 0;
+  int nnodes = 0, nelems = 0;
+  std::vector<int> conn, bcs;
+  std::vector<double> xpts, data, loads;
+  gf_options opt{};
+  size_t bytes = 0;  // estimated device footprint
+  Clock::time_point arrival;
+};
+
+struct Result {
+  long long id = 0;
+  std::vector<double> x;
+  gf_solve_info info{};
+  double latency_s = 0.0, queue_s = 0.0;
+  int status = GF_OK;
+};
+
+}  // namespace
+
+struct gf_server_s {
+  int device = 0;
+  size_t budget = 0;
+  std::atomic<size_t> in_use{0};
+  std::mutex mu;
+  std::condition_variable cv;      // queue / budget changes
+  std::deque<Request> queue;
+  std::deque<Result> done;
+  std::vector<std::thread> workers;
+  bool stopping = false;
+  int active = 0;
+  size_t peak_bytes = 0;
+  long long completed = 0, failed = 0;
+  std::vector<double> latencies;
+  Clock::time_point t0 = Clock::now();
+};
+
+namespace {
+
+// device bytes of one problem: K and ILU(k) values (ILU fill estimated as
+// (k+1) x nnzb_K, capped at 4x), FP32 copies in mixed mode, the Krylov basis,
+// index arrays and mesh
+size_t estimate_bytes(const Request& r) {
+  const double nnzb_k = 9.0 * r.nnodes;
+  const double fill = std::min(4.0, 1.0 + 0.4 * r.opt.lev_fill);
+  double b = nnzb_k * 288.0 * (1.0 + fill);
+  if (r.opt.precision == 1) b *= 1.5;
+  b += 48.0 * r.nnodes * (r.opt.restart + 8);
+  b += nnzb_k * 12.0 + r.nelems * 96.0 + r.nnodes * 64.0;
+  return (size_t)(b * 1.1);
+}
+
+void worker_main(gf_server_s* s) {
+  gf_context ctx = nullptr;
+  if (gf_context_create(s->device, &ctx) != GF_OK) return;
+  for (;;) {
+    Request req;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      // admit the oldest request once its footprint fits the budget
+      s->cv.wait(lk, [&] {
+        return s->stopping ||
+               (!s->queue.empty() &&
+                (s->in_use + s->queue.front().bytes <= s->budget ||
+                 s->active == 0));
+      });
+      if (s->stopping && s->queue.empty()) break;
+      req = std::move(s->queue.front());
+      s->queue.pop_front();
+      s->in_use += req.bytes;
+      s->active++;
+      s->peak_bytes = std::max(s->peak_bytes, (size_t)s->in_use);
+    }
+    Result res;
+    res.id = req.id;
+    const auto t_start = Clock::now();
+    res.queue_s = std::chrono::duration<double>(t_start - req.arrival).count();
+    gf_problem p = nullptr;
+    int st = gf_problem_create(ctx, req.nnodes, req.nelems, req.conn.data(),
+                               req.xpts.data(), req.data.data(),
+                               (int)req.bcs.size(), req.bcs.data(), &req.opt,
+                               &p);
+    if (st == GF_OK) st = gf_problem_assemble(p);
+    if (st == GF_OK) st = gf_problem_factor(p);
+    if (st == GF_OK) {
+      res.x.resize(6 * (size_t)req.nnodes);
+      st = gf_problem_solve(p, req.loads.data(), res.x.data(), 0, &res.info);
+    }
+    if (p) gf_problem_destroy(p);
+    res.status = st;
+    res.latency_s =
+        std::chrono::duration<double>(Clock::now() - req.arrival).count();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->in_use -= req.bytes;
+      s->active--;
+      if (st == GF_OK)
+        s->completed++;
+      else
+        s->failed++;
+      s->latencies.push_back(res.latency_s);
+      s->done.push_back(std::move(res));
+    }
+    s->cv.notify_all();
+  }
+  gf_context_destroy(ctx);
+}
+
+double percentile(std::vector<double> v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  const double pos = q * (v.size() - 1);
+  const size_t i = (size_t)pos;
+  const double f = pos - i;
+  return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int gf_server_create(int device, double budget_gb, int slots,
+                     gf_server* out) {
+  if (!out || slots < 1 || slots > 64 || budget_gb <= 0.0) return GF_ERR_INVALID;
+  auto* s = new gf_server_s;
+  s->device = device;
+  s->budget = (size_t)(budget_gb * 1e9);
+  for (int i = 0; i < slots; i++) s->workers.emplace_back(worker_main, s);
+  *out = s;
+  return GF_OK;
+}
+
+int gf_server_submit(gf_server s, long long id, int nnodes, int nelems,
+                     const int* conn, const double* xpts, const double* data,
+                     int nbcs, const int* bcs, const double* loads,
+                     const gf_options* opt) {
+  if (!s || nnodes <= 0 || nelems <= 0 || !conn || !xpts || !data || !loads ||
+      nbcs < 0 || (nbcs && !bcs))
+    return GF_ERR_INVALID;
+  Request r;
+  r.id = id;
+  r.nnodes = nnodes;
+  r.nelems = nelems;
+  r.conn.assign(conn, conn + 4 * (size_t)nelems);
+  r.xpts.assign(xpts, xpts + 3 * (size_t)nnodes);
+  r.data.assign(data, data + 4 * (size_t)nelems);
+  r.bcs.assign(bcs, bcs + nbcs);
+  r.loads.assign(loads, loads + 6 * (size_t)nnodes);
+  if (opt)
+    r.opt = *opt;
+  else
+    gf_options_default(&r.opt);
+  r.bytes = estimate_bytes(r);
+  r.arrival = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r.bytes > s->budget) return GF_ERR_OVER_BUDGET;
+    s->queue.push_back(std::move(r));
+  }
+  s->cv.notify_all();
+  return GF_OK;
+}
+
+// Pops one finished request; returns 1 if one was returned, 0 if none is
+// ready.  x must hold 6*nnodes doubles of that request (NULL to discard).
+int gf_server_poll(gf_server s, long long* id, double* x, size_t x_len,
+                   gf_solve_info* info, double* latency_s) {
+  if (!s) return GF_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->done.empty()) return 0;
+  Result r = std::move(s->done.front());
+  s->done.pop_front();
+  if (id) *id = r.id;
+  if (x && x_len >= r.x.size())
+    std::copy(r.x.begin(), r.x.end(), x);
+  if (info) {
+    *info = r.info;
+    info->status = r.status;
+  }
+  if (latency_s) *latency_s = r.latency_s;
+  return 1;
+}
+
+// stats: [0] completed [1] failed [2] queued [3] active [4] p50 latency s
+// [5] p99 latency s [6] peak bytes admitted [7] elapsed s
+int gf_server_stats(gf_server s, double* stats) {
+  if (!s || !stats) return GF_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  stats[0] = (double)s->completed;
+  stats[1] = (double)s->failed;
+  stats[2] = (double)s->queue.size();
+  stats[3] = (double)s->active;
+  stats[4] = percentile(s->latencies, 0.50);
+  stats[5] = percentile(s->latencies, 0.99);
+  stats[6] = (double)s->peak_bytes;
+  stats[7] = std::chrono::duration<double>(Clock::now() - s->t0).count();
+  return GF_OK;
+}
+
+// waits until every submitted request has finished
+int gf_server_drain(gf_server s) {
+  if (!s) return GF_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv.wait(lk, [&] { return s->queue.empty() && s->active == 0; });
+  return GF_OK;
+}
+
+int gf_server_destroy(gf_server s) {
+  if (!s) return GF_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv.notify_all();
+  for (auto& t : s->workers) t.join();
+  delete s;
+  return GF_OK;
+}
+
+}  // extern "C"

@@ -44,6 +44,7 @@ enum {
 typedef struct gf_context_s* gf_context;
 typedef struct gf_ilu_s* gf_ilu;
 typedef struct gf_problem_s* gf_problem;
+typedef struct gf_server_s* gf_server;
 
 /* ---- context (replaces the implicit default stream + exit() on error of
  *      include/cuda_utils.h:33-83) ---- */
@@ -209,6 +210,25 @@ int gf_problem_device_arrays(gf_problem p, const int** krowp,
                              const int** mrowp, const int** mcols,
                              double** mvals, const int** perm);
 int gf_problem_host_perm(gf_problem p, int* perm);
+
+/* ---- serving loop (no reference counterpart: the served workload's request
+ *      loop).  Worker slots, each with its own stream, admit queued requests
+ *      FIFO when their estimated device footprint fits the HBM budget and run
+ *      gf_problem_create/assemble/factor/solve; device work of concurrent
+ *      requests overlaps.  Thread-safe. ---- */
+int gf_server_create(int device, double budget_gb, int slots, gf_server* out);
+/* copies the request; GF_ERR_OVER_BUDGET if it can never fit */
+int gf_server_submit(gf_server s, long long id, int nnodes, int nelems,
+                     const int* conn, const double* xpts, const double* data,
+                     int nbcs, const int* bcs, const double* loads,
+                     const gf_options* opt);
+/* 1 and the result of one finished request, or 0 if none is ready */
+int gf_server_poll(gf_server s, long long* id, double* x, size_t x_len,
+                   gf_solve_info* info, double* latency_s);
+/* [completed, failed, queued, active, p50 s, p99 s, peak bytes, elapsed s] */
+int gf_server_stats(gf_server s, double* stats);
+int gf_server_drain(gf_server s);
+int gf_server_destroy(gf_server s);
 
 #ifdef __cplusplus
 }
