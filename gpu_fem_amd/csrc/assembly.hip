@@ -60,16 +60,17 @@ __device__ __forceinline__ void element_geometry(ElemLDS& s, int lane) {
 
 __device__ __forceinline__ void element_bd(ElemLDS& s, int lane) {
   if (lane < 24) {
-    double col[4][9];
-    mitc4::b_column(lane, s.g, col);
-#pragma unroll
+    mitc4::BColumnPrep prep;
+    mitc4::b_column_prep(lane, s.g, prep);
+#pragma unroll 1
     for (int q = 0; q < 4; q++) {
-      double st[9];
-      mitc4::stress(s.data, col[q], st);
+      double col[9], st[9];
+      mitc4::b_column_q(lane, q, s.g, prep, col);
+      mitc4::stress(s.data, col, st);
       const double sc = s.g.scale[q];
 #pragma unroll
       for (int k = 0; k < 9; k++) {
-        s.B[q][k][lane] = col[q][k];
+        s.B[q][k][lane] = col[k];
         s.D[q][k][lane] = sc * st[k];
       }
     }

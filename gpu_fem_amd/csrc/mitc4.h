@@ -254,118 +254,142 @@ __device__ __forceinline__ void geom_quad(int q, const double X[12],
   }
 }
 
-// Column j (DOF 6a + k) of B_q for the four quadrature points: out[q][s].
-__device__ __forceinline__ void b_column(int j, const Geom& g,
-                                         double out[4][9]) {
+// Shape function of node a and its derivatives at (xi, eta):
+// N_a = (1 + xa xi)(1 + ea eta) / 4 with (xa, ea) the node's corner.
+__device__ __forceinline__ void shape_a(int a, double xi, double eta,
+                                        double& N, double& Nx, double& Ne) {
+  const double xa = (a & 1) ? 1.0 : -1.0, ea = (a & 2) ? 1.0 : -1.0;
+  const double fx = 0.5 * (1.0 + xa * xi), fe = 0.5 * (1.0 + ea * eta);
+  N = fx * fe;
+  Nx = 0.5 * xa * fe;
+  Ne = 0.5 * ea * fx;
+}
+
+// Quadrature-point independent part of column j (DOF 6a + k): the nine
+// tying-strain derivatives and the drill contributions at the four nodes.
+struct BColumnPrep {
+  double ety[9];
+  double drill_node[4];
+};
+
+__device__ __forceinline__ void b_column_prep(int j, const Geom& g,
+                                              BColumnPrep& p) {
   const int a = j / 6, k = j % 6;
   const bool rot = k >= 3;
   const int kk = rot ? k - 3 : k;
-  // tying strain derivatives ety[9] for this DOF
-  double ety[9];
-  {
-    double N[4], Nx[4], Ne[4];
-    double loc[5][3];  // N_a, N_a,xi, N_a,eta at each tying location
-    for (int l = 0; l < 5; l++) {
-      double xi, eta;
-      tying_loc(l, xi, eta);
-      shape(xi, eta, N, Nx, Ne);
-      loc[l][0] = N[a];
-      loc[l][1] = Nx[a];
-      loc[l][2] = Ne[a];
-    }
-    if (!rot) {
-      ety[0] = loc[0][1] * g.tXxi[0][kk];
-      ety[1] = loc[1][1] * g.tXxi[1][kk];
-      ety[2] = loc[2][2] * g.tXeta[2][kk];
-      ety[3] = loc[3][2] * g.tXeta[3][kk];
-      ety[4] = 0.5 * (loc[4][1] * g.tXeta[4][kk] + loc[4][2] * g.tXxi[4][kk]);
-      ety[5] = 0.5 * loc[2][2] * g.tn0[2][kk];
-      ety[6] = 0.5 * loc[3][2] * g.tn0[3][kk];
-      ety[7] = 0.5 * loc[0][1] * g.tn0[0][kk];
-      ety[8] = 0.5 * loc[1][1] * g.tn0[1][kk];
-    } else {
-      double c[3];
-      ety[0] = ety[1] = ety[2] = ety[3] = ety[4] = 0.0;
-      cross(g.fn[a], g.tXeta[2], c);
-      ety[5] = 0.5 * loc[2][0] * c[kk];
-      cross(g.fn[a], g.tXeta[3], c);
-      ety[6] = 0.5 * loc[3][0] * c[kk];
-      cross(g.fn[a], g.tXxi[0], c);
-      ety[7] = 0.5 * loc[0][0] * c[kk];
-      cross(g.fn[a], g.tXxi[1], c);
-      ety[8] = 0.5 * loc[1][0] * c[kk];
-    }
+  double N[5], Nx[5], Ne[5];  // node a's shape functions at the 5 locations
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    double xi, eta;
+    tying_loc(l, xi, eta);
+    shape_a(a, xi, eta, N[l], Nx[l], Ne[l]);
   }
-  // drill: translational part needs N_a,xi / N_a,eta at the node points
-  double drill_node[4];
+  double* ety = p.ety;
   if (!rot) {
-    for (int b = 0; b < 4; b++) {
-      // N_a,xi at node b is +-1/2 when a and b share the eta row, else 0
-      const double xa = (a % 2) ? 1.0 : -1.0, ea = (a / 2) ? 1.0 : -1.0;
-      const double nxi = ((a / 2) == (b / 2)) ? 0.5 * xa : 0.0;
-      const double neta = ((a % 2) == (b % 2)) ? 0.5 * ea : 0.0;
-      drill_node[b] = 0.5 * (nxi * g.dv[b][kk] + neta * g.dw[b][kk]);
-    }
+    ety[0] = Nx[0] * g.tXxi[0][kk];
+    ety[1] = Nx[1] * g.tXxi[1][kk];
+    ety[2] = Ne[2] * g.tXeta[2][kk];
+    ety[3] = Ne[3] * g.tXeta[3][kk];
+    ety[4] = 0.5 * (Nx[4] * g.tXeta[4][kk] + Ne[4] * g.tXxi[4][kk]);
+    ety[5] = 0.5 * Ne[2] * g.tn0[2][kk];
+    ety[6] = 0.5 * Ne[3] * g.tn0[3][kk];
+    ety[7] = 0.5 * Nx[0] * g.tn0[0][kk];
+    ety[8] = 0.5 * Nx[1] * g.tn0[1][kk];
+  } else {
+    double c[3];
+    ety[0] = ety[1] = ety[2] = ety[3] = ety[4] = 0.0;
+    cross(g.fn[a], g.tXeta[2], c);
+    ety[5] = 0.5 * N[2] * c[kk];
+    cross(g.fn[a], g.tXeta[3], c);
+    ety[6] = 0.5 * N[3] * c[kk];
+    cross(g.fn[a], g.tXxi[0], c);
+    ety[7] = 0.5 * N[0] * c[kk];
+    cross(g.fn[a], g.tXxi[1], c);
+    ety[8] = 0.5 * N[1] * c[kk];
   }
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const double xi = (q % 2 == 0) ? -kRt3 : kRt3;
-    const double eta = (q / 2 == 0) ? -kRt3 : kRt3;
-    double N[4], Nx[4], Ne[4];
-    shape(xi, eta, N, Nx, Ne);
-    // interpolated tying strains (basis.h:314-368)
-    double gt[5];
-    gt[0] = 0.5 * (1.0 - eta) * ety[0] + 0.5 * (1.0 + eta) * ety[1];
-    gt[1] = 0.5 * (1.0 - xi) * ety[2] + 0.5 * (1.0 + xi) * ety[3];
-    gt[2] = ety[4];
-    gt[3] = 0.5 * (1.0 - xi) * ety[5] + 0.5 * (1.0 + xi) * ety[6];
-    gt[4] = 0.5 * (1.0 - eta) * ety[7] + 0.5 * (1.0 + eta) * ety[8];
-    const double* cf = g.coef[q];
-    double e[5];
-    for (int m = 0; m < 5; m++)
-      e[m] = cf[5 * m] * gt[0] + cf[5 * m + 1] * gt[1] + cf[5 * m + 2] * gt[2] +
-             cf[5 * m + 3] * gt[3] + cf[5 * m + 4] * gt[4];
-    // bending u1x(kk2, jj2) derivative
-    const double* A = g.A[q];
-    const double* Z = g.Z[q];
+  for (int b = 0; b < 4; b++) {
+    if (rot) {
+      p.drill_node[b] = 0.0;
+      continue;
+    }
+    // N_a,xi at node b is +-1/2 when a and b share the eta row, else 0
+    const double xa = (a & 1) ? 1.0 : -1.0, ea = (a & 2) ? 1.0 : -1.0;
+    const double nxi = ((a >> 1) == (b >> 1)) ? 0.5 * xa : 0.0;
+    const double neta = ((a & 1) == (b & 1)) ? 0.5 * ea : 0.0;
+    p.drill_node[b] = 0.5 * (nxi * g.dv[b][kk] + neta * g.dw[b][kk]);
+  }
+}
+
+// Column j of B_q at quadrature point q: out[s], s = 0..8.
+__device__ __forceinline__ void b_column_q(int j, int q, const Geom& g,
+                                           const BColumnPrep& p,
+                                           double out[9]) {
+  const int a = j / 6, k = j % 6;
+  const bool rot = k >= 3;
+  const int kk = rot ? k - 3 : k;
+  const double xi = (q & 1) ? kRt3 : -kRt3;
+  const double eta = (q & 2) ? kRt3 : -kRt3;
+  double Na, Nxa, Nea;
+  shape_a(a, xi, eta, Na, Nxa, Nea);
+  const double* ety = p.ety;
+  // interpolated tying strains (basis.h:314-368)
+  double gt[5];
+  gt[0] = 0.5 * (1.0 - eta) * ety[0] + 0.5 * (1.0 + eta) * ety[1];
+  gt[1] = 0.5 * (1.0 - xi) * ety[2] + 0.5 * (1.0 + xi) * ety[3];
+  gt[2] = ety[4];
+  gt[3] = 0.5 * (1.0 - xi) * ety[5] + 0.5 * (1.0 + xi) * ety[6];
+  gt[4] = 0.5 * (1.0 - eta) * ety[7] + 0.5 * (1.0 + eta) * ety[8];
+  const double* cf = g.coef[q];
+  double e[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++)
+    e[m] = cf[5 * m] * gt[0] + cf[5 * m + 1] * gt[1] + cf[5 * m + 2] * gt[2] +
+           cf[5 * m + 3] * gt[3] + cf[5 * m + 4] * gt[4];
+  // bending: derivative of u1x(k2, j2)
+  const double* A = g.A[q];
+  const double* Z = g.Z[q];
+  double u00, u11, u01, u10;
+  if (!rot) {
+    const double c0 = Nxa * Z[0] + Nea * Z[3];
+    const double c1 = Nxa * Z[1] + Nea * Z[4];
+    const double t0k = g.T[q][kk], t1k = g.T[q][3 + kk];
+    u00 = c0 * t0k;
+    u11 = c1 * t1k;
+    u01 = c1 * t0k;
+    u10 = c0 * t1k;
+  } else {
+    const double c0 = Nxa * A[0] + Nea * A[3] + Na * Z[6];
+    const double c1 = Nxa * A[1] + Nea * A[4] + Na * Z[7];
     const double t0[3] = {g.T[q][0], g.T[q][1], g.T[q][2]};
     const double t1[3] = {g.T[q][3], g.T[q][4], g.T[q][5]};
-    double u00, u11, u01, u10;
-    if (!rot) {
-      const double c0 = Nx[a] * Z[0] + Ne[a] * Z[3];  // column 0
-      const double c1 = Nx[a] * Z[1] + Ne[a] * Z[4];  // column 1
-      u00 = c0 * t0[kk];
-      u11 = c1 * t1[kk];
-      u01 = c1 * t0[kk];
-      u10 = c0 * t1[kk];
-    } else {
-      const double c0 = Nx[a] * A[0] + Ne[a] * A[3] + N[a] * Z[6];
-      const double c1 = Nx[a] * A[1] + Ne[a] * A[4] + N[a] * Z[7];
-      double f0[3], f1[3];
-      cross(g.fn[a], t0, f0);
-      cross(g.fn[a], t1, f1);
-      u00 = c0 * f0[kk];
-      u11 = c1 * f1[kk];
-      u01 = c1 * f0[kk];
-      u10 = c0 * f1[kk];
-    }
-    double et;
-    if (!rot) {
-      et = N[0] * drill_node[0] + N[1] * drill_node[1] + N[2] * drill_node[2] +
-           N[3] * drill_node[3];
-    } else {
-      et = -N[a] * g.dn[a][kk];
-    }
-    out[q][0] = e[0];
-    out[q][1] = e[1];
-    out[q][2] = e[2];
-    out[q][3] = u00;
-    out[q][4] = u11;
-    out[q][5] = u01 + u10;
-    out[q][6] = e[3];
-    out[q][7] = e[4];
-    out[q][8] = et;
+    double f0[3], f1[3];
+    cross(g.fn[a], t0, f0);
+    cross(g.fn[a], t1, f1);
+    u00 = c0 * f0[kk];
+    u11 = c1 * f1[kk];
+    u01 = c1 * f0[kk];
+    u10 = c0 * f1[kk];
   }
+  double et;
+  if (!rot) {
+    double Nb[4], d1, d2;
+#pragma unroll
+    for (int b = 0; b < 4; b++) shape_a(b, xi, eta, Nb[b], d1, d2);
+    et = Nb[0] * p.drill_node[0] + Nb[1] * p.drill_node[1] +
+         Nb[2] * p.drill_node[2] + Nb[3] * p.drill_node[3];
+  } else {
+    et = -Na * g.dn[a][kk];
+  }
+  out[0] = e[0];
+  out[1] = e[1];
+  out[2] = e[2];
+  out[3] = u00;
+  out[4] = u11;
+  out[5] = u01 + u10;
+  out[6] = e[3];
+  out[7] = e[4];
+  out[8] = et;
 }
 
 // S = C E for one material point (a2d/a2disotropicshellstress.h:13-51).
