@@ -37,16 +37,19 @@ def mesh(nx, ccw=False):
                    load="nodal_sine", load_mag=1.0, ccw=ccw)
 
 
-@pytest.mark.parametrize("rcm,k", [(0, 0), (1, 2)])
+# On this plate (free edge at y = Ly, t/L = 1/400) GMRES(100) stagnates near
+# a relative residual of 0.9 with ILU(0) or ILU(1) in either order, on the
+# oracle as on the device; RCM with ILU(2) or ILU(3) converges.
+@pytest.mark.parametrize("rcm,k", [(1, 2), (1, 3)])
 def test_gmres_flow_matches_oracle(rcm, k):
     print(f"[gpu] linear_static app nx=16 rcm={rcm} k={k}", flush=True)
-    info, x = run_app(16, k, rcm, 100, 1e-11)
+    info, x = run_app(16, k, rcm, 100, 1e-10)
     m = mesh(16)
     s = oracle_problem(m, "rcm" if rcm else "natural", k)
     lu, _ = oracle.bsr_ilu(s["mrp"], s["mcp"], s["mv"])
     b = oracle.permute_vec(s["perm"], oracle.apply_bcs_vec(m.bcs, m.loads))
     xp, st, _ = oracle.gmres(s["krp"], s["kcp"], s["kv"], s["mrp"], s["mcp"],
-                             lu, b, restart=100, rtol=1e-11, max_iters=5000)
+                             lu, b, restart=100, rtol=1e-10, max_iters=5000)
     assert st == 0
     xo = oracle.permute_vec(np.argsort(s["perm"]).astype(np.int32), xp)
     assert info["nnzb_K"] == s["krp"][-1] and info["nnzb_ILU"] == s["mrp"][-1]
