@@ -32,7 +32,9 @@ def requests(seed=0, n=10):
     return out
 
 
-OPTS = dict(reorder=0, lev_fill=2, restart=80, rtol=1e-9, max_iters=20000)
+# AMD order with ILU(3) converges on every mesh of this mix (thin plates
+# and cylinders); natural or RCM order with ILU(2) stagnates on several.
+OPTS = dict(reorder=2, lev_fill=3, restart=80, rtol=1e-9, max_iters=20000)
 
 
 def test_mix_equals_alone(gpu_ctx):
