@@ -80,6 +80,9 @@ _SIGS = {
     "gf_bsr_spmv_f64": [_P, _I, _P, _P, _P, _P, _P],
     "gf_bsr_spmv_f32": [_P, _I, _P, _P, _P, _P, _P],
     "gf_convert_f64_f32": [_P, ctypes.c_size_t, _P, _P],
+    "gf_bsr_resid_f64": [_P, _I, _P, _P, _P, _P, _P,
+                         ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double)],
     "gf_ilu_create": [_P, _I, _P, _P, ctypes.POINTER(_P)],
     "gf_ilu_destroy": [_P],
     "gf_ilu_info": [_P, _PI, _PI, _PI],

@@ -121,6 +121,7 @@ int main(int argc, char** argv) {
     auto rhs = bsr_pre_solve<DeviceVec<T>>(kmat, loads, soln);
     auto stats = solvers::gmres(kmat, precond, rhs, soln, restart, 20000,
                                 tolerance);
+    const T max_resid = solvers::get_resid(kmat, rhs, soln);
     bsr_post_solve<DeviceVec<T>>(kmat, loads, soln);
 
     auto h = soln.createHostVec();
@@ -128,8 +129,10 @@ int main(int argc, char** argv) {
     std::fwrite(h.getPtr(), sizeof(T), h.getSize(), f);
     std::fclose(f);
     std::printf("{\"path\": \"gmres\", \"nnodes\": %d, \"nnzb_K\": %d, "
-                "\"nnzb_ILU\": %d, \"iterations\": %d, \"rel_resid\": %.6e}\n",
-                nnodes, kmat_nnzb, nnzb, stats.iterations, stats.rel_resid);
+                "\"nnzb_ILU\": %d, \"iterations\": %d, \"rel_resid\": %.6e, "
+                "\"max_resid\": %.6e}\n",
+                nnodes, kmat_nnzb, nnzb, stats.iterations, stats.rel_resid,
+                max_resid);
   } catch (const Error& e) {
     std::fprintf(stderr, "linear_static: %s (code %d)\n", e.what(), e.code);
     return 1;

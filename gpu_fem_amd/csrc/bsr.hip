@@ -110,6 +110,95 @@ __global__ void __launch_bounds__(256)
   y[6 * (size_t)row + r] = acc;
 }
 
+// r = A x - b; per workgroup: the entry of largest magnitude (first index on
+// ties) and the sum of squares
+__global__ void __launch_bounds__(256)
+    resid_partial_kernel(int nnodes, const int* __restrict__ rowp,
+                         const int* __restrict__ cols,
+                         const double* __restrict__ vals,
+                         const double* __restrict__ x,
+                         const double* __restrict__ b, double* pmax,
+                         long long* pidx, double* pss) {
+  __shared__ double smax[256], sss[256];
+  __shared__ long long sidx[256];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int row = (blockIdx.x * (blockDim.x / 64) + wave) * 10 + lane / 6;
+  const int r = lane % 6;
+  double v = 0.0;
+  long long idx = 0x7fffffffffffffffll;
+  if (lane < 60 && row < nnodes) {
+    double acc = 0.0;
+    for (int k = rowp[row]; k < rowp[row + 1]; k++) {
+      const double* a = vals + 36 * (size_t)k + 6 * r;
+      const double* xb = x + 6 * (size_t)cols[k];
+#pragma unroll
+      for (int c = 0; c < 6; c++) acc += a[c] * xb[c];
+    }
+    idx = 6 * (long long)row + r;
+    v = acc - b[idx];
+  }
+  smax[threadIdx.x] = v;
+  sidx[threadIdx.x] = idx;
+  sss[threadIdx.x] = v * v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const int o = threadIdx.x + s;
+      const double a0 = fabs(smax[threadIdx.x]), a1 = fabs(smax[o]);
+      if (a1 > a0 || (a1 == a0 && sidx[o] < sidx[threadIdx.x])) {
+        smax[threadIdx.x] = smax[o];
+        sidx[threadIdx.x] = sidx[o];
+      }
+      sss[threadIdx.x] += sss[o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmax[blockIdx.x] = smax[0];
+    pidx[blockIdx.x] = sidx[0];
+    pss[blockIdx.x] = sss[0];
+  }
+}
+
+// one workgroup folds the partials in a fixed order
+__global__ void __launch_bounds__(256)
+    resid_final_kernel(int nparts, const double* __restrict__ pmax,
+                       const long long* __restrict__ pidx,
+                       const double* __restrict__ pss, double* out) {
+  __shared__ double smax[256], sss[256];
+  __shared__ long long sidx[256];
+  double m = 0.0, ss = 0.0;
+  long long id = 0x7fffffffffffffffll;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    const double a = fabs(pmax[i]), am = fabs(m);
+    if (a > am || (a == am && pidx[i] < id)) {
+      m = pmax[i];
+      id = pidx[i];
+    }
+    ss += pss[i];
+  }
+  smax[threadIdx.x] = m;
+  sidx[threadIdx.x] = id;
+  sss[threadIdx.x] = ss;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const int o = threadIdx.x + s;
+      const double a0 = fabs(smax[threadIdx.x]), a1 = fabs(smax[o]);
+      if (a1 > a0 || (a1 == a0 && sidx[o] < sidx[threadIdx.x])) {
+        smax[threadIdx.x] = smax[o];
+        sidx[threadIdx.x] = sidx[o];
+      }
+      sss[threadIdx.x] += sss[o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smax[0];
+    out[1] = sqrt(sss[0]);
+  }
+}
+
 __global__ void convert_kernel(size_t n, const double* __restrict__ in,
                                float* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,6 +275,39 @@ int gf_bsr_spmv_f32(gf_context ctx, int nnodes, const int* rowp,
   hipLaunchKernelGGL(gf::spmv_kernel<float>, dim3(gf::div_up(nnodes, 40)),
                      dim3(256), 0, ctx->stream, nnodes, rowp, cols, vals, x, y);
   return gf::check_launch(ctx, "spmv_kernel<float>");
+}
+
+int gf_bsr_resid_f64(gf_context ctx, int nnodes, const int* rowp,
+                     const int* cols, const double* vals, const double* x,
+                     const double* b, double* max_entry, double* norm2) {
+  if (!ctx || nnodes <= 0 || !rowp || !cols || !vals || !x || !b)
+    return gf::set_error(ctx, GF_ERR_INVALID, "bsr_resid: bad args");
+  const int nblk = (int)gf::div_up(nnodes, 40);
+  char* buf = nullptr;
+  const size_t part = sizeof(double) * (size_t)nblk;
+  GF_HIP(ctx, hipMalloc(&buf, 3 * part + 2 * sizeof(double)));
+  double* pmax = (double*)buf;
+  long long* pidx = (long long*)(buf + part);
+  double* pss = (double*)(buf + 2 * part);
+  double* out = (double*)(buf + 3 * part);
+  hipLaunchKernelGGL(gf::resid_partial_kernel, dim3(nblk), dim3(256), 0,
+                     ctx->stream, nnodes, rowp, cols, vals, x, b, pmax, pidx,
+                     pss);
+  hipLaunchKernelGGL(gf::resid_final_kernel, dim3(1), dim3(256), 0,
+                     ctx->stream, nblk, pmax, pidx, pss, out);
+  double h[2] = {0.0, 0.0};
+  int st = gf::check_launch(ctx, "resid_kernel");
+  if (st == GF_OK)
+    st = gf::check_hip(ctx, hipMemcpyAsync(h, out, sizeof(h),
+                                           hipMemcpyDeviceToHost, ctx->stream),
+                       "resid copy");
+  if (st == GF_OK)
+    st = gf::check_hip(ctx, hipStreamSynchronize(ctx->stream), "resid sync");
+  hipFree(buf);
+  if (st != GF_OK) return st;
+  if (max_entry) *max_entry = h[0];
+  if (norm2) *norm2 = h[1];
+  return GF_OK;
 }
 
 int gf_convert_f64_f32(gf_context ctx, size_t n, const double* in,

@@ -11,16 +11,17 @@
 // Schedules
 //   factorisation: rows grouped by dependency level on the host (the levels
 //     of the factorisation equal those of the L sweep); one launch per level.
-//   sweeps: one persistent launch per sweep.  A wavefront takes the next row
-//     in topological order from a 64-bit counter, prefetches the row's
-//     blocks, waits on per-row ready flags of the rows it depends on
-//     (agent-scope acquire, bounded spin), reduces across lanes, solves the
-//     6x6 diagonal block with lane shuffles, stores, and publishes its flag
-//     (agent-scope release).  Rows are handed out in dependency order, so
-//     every row waited on is already held by a running wavefront and the
-//     sweep cannot deadlock; flags carry a sweep epoch, so they are never
-//     cleared.  The level-by-level sweep stays available
-//     (GPUFEM_TRSV=level) as an independent schedule for parity checks.
+//   sweeps: one persistent launch per sweep, synchronised through the data:
+//     the output vector is pre-filled with a signalling-NaN sentinel, each
+//     entry is written once with a relaxed agent-scope atomic store, and a
+//     row polls the entries it needs with relaxed agent-scope atomic loads
+//     until none holds the sentinel.  No fence and no flag array: each entry
+//     validates itself, so a dependency hop costs one store and one load
+//     round trip, without L2 write-back or invalidation.  Rows are handed out
+//     in level order, in chunks, from a 64-bit counter, so every row waited
+//     on is already held by a running wavefront and the sweep cannot
+//     deadlock.  The level-by-level sweep stays available (GPUFEM_TRSV=level)
+//     as an independent schedule for parity checks.
 #include <algorithm>
 #include <cstdlib>
 #include <string>
@@ -38,13 +39,11 @@ struct gf_ilu_s {
   int* pivot = nullptr;                    // first zero-pivot row (device)
   int max_row_len = 0;
   // sync-free sweep state
-  int* flags = nullptr;                    // per-row epoch of completion
-  int *lchain = nullptr, *uchain = nullptr; // chain offsets in sweep order
-  int nlchain = 0, nuchain = 0;
-  unsigned long long* counter = nullptr;   // row dispenser (monotonic)
+  int *lchunk = nullptr, *uchunk = nullptr; // chunk offsets into l/urows
+  int nlchunk = 0, nuchunk = 0;
+  unsigned long long* counter = nullptr;   // chunk dispenser (monotonic)
   int* err = nullptr;                      // set when a bounded wait expires
   unsigned long long base = 0;             // counter value at sweep start
-  int epoch = 0;
   int grid = 0;                            // workgroups per sweep launch
   bool level_sweeps = false;
 };
@@ -197,96 +196,169 @@ __global__ void __launch_bounds__(256)
 
 
 constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
-constexpr int kPrefetchBlocks = 24;       // L/U blocks held in registers
-constexpr unsigned kSpinLimit = 1u << 22; // bounded wait per dependency
+constexpr int kSweepIter = 4;             // block batches held in registers
+constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
+constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
-constexpr int kChainCache = 4;           // recent rows of a chain in registers
+// Sentinel values: signalling NaNs.  Arithmetic never produces a signalling
+// NaN (results are quiet), so an output entry that holds the sentinel has not
+// been written yet in the current sweep.
+template <typename T> struct Sentinel;
+template <> struct Sentinel<double> {
+  using Bits = unsigned long long;
+  static constexpr Bits value = 0x7FF4A5A5A5A5A5A5ull;
+};
+template <> struct Sentinel<float> {
+  using Bits = unsigned int;
+  static constexpr Bits value = 0x7FA5A5A5u;
+};
 
-// Rows are handed out as chains: maximal runs of consecutive rows (in sweep
-// order) where each row depends on the one before it.  A wavefront solves its
-// chain row after row; the dependencies on the last kChainCache rows of its
-// own chain come from registers, every other dependency through the flags.
-// Chains are dispensed in sweep order and every dependency of a row lies in
-// its own chain or in an earlier one, so no wavefront waits on a chain that
-// has not been dispensed.
+template <typename T>
+__device__ __forceinline__ typename Sentinel<T>::Bits load_bits(const T* p) {
+  using B = typename Sentinel<T>::Bits;
+  return __hip_atomic_load(reinterpret_cast<const B*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void store_value(T* p, T v) {
+  using B = typename Sentinel<T>::Bits;
+  __hip_atomic_store(reinterpret_cast<B*>(p), __builtin_bit_cast(B, v),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ void fill_sentinel_kernel(size_t n, T* x) {
+  using B = typename Sentinel<T>::Bits;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) reinterpret_cast<B*>(x)[i] = Sentinel<T>::value;
+}
+
+// Sync-free sweep.  The output vector is filled with the sentinel before the
+// launch; every output entry is written exactly once, with a relaxed
+// agent-scope atomic store, and a row reads each entry it depends on with
+// relaxed agent-scope atomic loads until the value is no longer the sentinel.
+// Each entry validates itself, so no fences or flags are needed.
+//
+// Rows are taken in level order, in chunks of rows of one level (chunk
+// boundaries from the host), from a monotonic 64-bit counter.  Every
+// dependency of a row lies in an earlier level, hence in a chunk handed out
+// earlier to a running wavefront, so the sweep cannot deadlock whatever the
+// number of resident workgroups.
+//
+// Lanes: slot t = lane / 6 (0..9) takes blocks k0+t, k0+t+10, ...; r =
+// lane % 6 is the scalar row; the 6x6 diagonal block is solved by lanes 0-5
+// with shuffles.  The last row solved by the wavefront is kept in registers
+// (rows of a chunk often depend on the row before).
+//
+// fill_next (lower sweep only, may be null): the entries of the row in the
+// upper sweep's output are set to the sentinel once the row's rhs is read,
+// which allows that output to alias the rhs.
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(kSweepThreads)
-    trsv_syncfree_kernel(int n, const int* __restrict__ rowp,
+    trsv_syncfree_kernel(const int* __restrict__ rowp,
                          const int* __restrict__ cols,
                          const int* __restrict__ diag,
-                         const int* __restrict__ chain_ptr, int nchains,
-                         const T* __restrict__ lu, const T* __restrict__ b,
-                         T* x, int* flags, int epoch,
-                         unsigned long long* counter, unsigned long long base,
-                         int* err) {
+                         const int* __restrict__ order,
+                         const int* __restrict__ chunk_ptr, int nchunks,
+                         const T* __restrict__ lu, const T* b, T* x,
+                         T* fill_next, unsigned long long* counter,
+                         unsigned long long base, int* err) {
+  using B = typename Sentinel<T>::Bits;
   const int lane = threadIdx.x % 64;
-  const int c = lane % 6;  // lanes 0..35: block entry (lane / 6, lane % 6)
-  const bool ent = lane < 36;
-  for (;;) {
-    unsigned long long t = 0;
-    if (lane == 0) t = atomicAdd(counter, 1ull);
-    t = __shfl(t, 0, 64);
-    const long long kc = (long long)(t - base);
-    if (kc >= nchains) break;
-    const int c0 = chain_ptr[kc], c1 = chain_ptr[kc + 1];
-    T xc[kChainCache];  // x[row - d][c] for d = 1..kChainCache
+  const int t = lane / 6, r = lane % 6;
+  const bool act = lane < 6 * kSlots;
+  int prev = -1;   // last row solved by this wavefront
+  T xprev[6];      // its solution, in every lane
 #pragma unroll
-    for (int d = 0; d < kChainCache; d++) xc[d] = T(0);
-    for (int tpos = c0; tpos < c1; tpos++) {
-      const int i = kLower ? tpos : n - 1 - tpos;
-      const int done = tpos - c0;  // rows of this chain already solved
+  for (int c = 0; c < 6; c++) xprev[c] = T(0);
+  for (;;) {
+    unsigned long long tk = 0;
+    if (lane == 0) tk = atomicAdd(counter, 1ull);
+    tk = __shfl(tk, 0, 64);
+    const long long kc = (long long)(tk - base);
+    if (kc >= nchunks) break;
+    const int p0 = chunk_ptr[kc], p1 = chunk_ptr[kc + 1];
+    for (int p = p0; p < p1; p++) {
+      const int i = order[p];
       const int di = diag[i];
       const int k0 = kLower ? rowp[i] : di + 1;
       const int k1 = kLower ? di : rowp[i + 1];
-      const int nb = k1 - k0;
-      T pre[kPrefetchBlocks];
-#pragma unroll
-      for (int q = 0; q < kPrefetchBlocks; q++)
-        pre[q] = (ent && q < nb) ? lu[36 * (size_t)(k0 + q) + lane] : T(0);
+      T s = (lane < 6) ? b[6 * (size_t)i + lane] : T(0);
+      if (kLower && fill_next && lane < 6)
+        reinterpret_cast<B*>(fill_next)[6 * (size_t)i + lane] =
+            Sentinel<T>::value;
       T drow[6];
 #pragma unroll
       for (int m = 0; m < 6; m++)
         drow[m] = (lane < 6) ? lu[36 * (size_t)di + 6 * lane + m] : T(0);
-      T s = (lane < 6) ? b[6 * (size_t)i + lane] : T(0);
-      // wait for dependencies outside the register cache
-      bool ext = false;
-      for (int q = lane; q < nb; q += 64) {
-        const int j = cols[k0 + q];
-        const int dist = kLower ? i - j : j - i;
-        if (dist <= done && dist <= kChainCache) continue;
-        ext = true;
+      T acc = T(0);
+      for (int q0 = k0; q0 < k1; q0 += kSlots * kSweepIter) {
+        T a[kSweepIter][6], xv[kSweepIter][6];
+        int jj[kSweepIter];
+#pragma unroll
+        for (int it = 0; it < kSweepIter; it++) {
+          const int q = q0 + kSlots * it + t;
+          const bool ok = act && q < k1;
+          jj[it] = ok ? cols[q] : -1;
+          const T* ap = lu + 36 * (size_t)(ok ? q : 0) + 6 * r;
+#pragma unroll
+          for (int c = 0; c < 6; c++) a[it][c] = ok ? ap[c] : T(0);
+        }
+        // all dependency loads first (one round trip), then re-poll the
+        // entries still holding the sentinel
+        B xb[kSweepIter][6];
+#pragma unroll
+        for (int it = 0; it < kSweepIter; it++) {
+          const int j = jj[it];
+#pragma unroll
+          for (int c = 0; c < 6; c++)
+            xb[it][c] = (j >= 0 && j != prev) ? load_bits(x + 6 * (size_t)j + c)
+                                              : B(0);
+        }
         unsigned spins = 0;
-        while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-          __builtin_amdgcn_s_sleep(1);
+        for (;;) {
+          bool pending = false;
+#pragma unroll
+          for (int it = 0; it < kSweepIter; it++)
+#pragma unroll
+            for (int c = 0; c < 6; c++)
+              pending |= xb[it][c] == Sentinel<T>::value;
+          if (!__any(pending)) break;
           if (++spins > kSpinLimit) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int it = 0; it < kSweepIter; it++)
+#pragma unroll
+              for (int c = 0; c < 6; c++) xb[it][c] = B(0);
             break;
           }
+          __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+          for (int it = 0; it < kSweepIter; it++)
+#pragma unroll
+            for (int c = 0; c < 6; c++)
+              if (xb[it][c] == Sentinel<T>::value)
+                xb[it][c] = load_bits(x + 6 * (size_t)jj[it] + c);
         }
+#pragma unroll
+        for (int it = 0; it < kSweepIter; it++)
+#pragma unroll
+          for (int c = 0; c < 6; c++)
+            xv[it][c] = jj[it] == prev && jj[it] >= 0
+                            ? xprev[c]
+                            : __builtin_bit_cast(T, xb[it][c]);
+#pragma unroll
+        for (int it = 0; it < kSweepIter; it++)
+#pragma unroll
+          for (int c = 0; c < 6; c++) acc += a[it][c] * xv[it][c];
       }
-      if (__any(ext)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      // partial products, lane (r, c): sum_q A[q][r][c] x[col_q][c]
-      T acc = 0;
-      auto xdep = [&](int q) -> T {
-        const int j = cols[k0 + q];
-        const int dist = kLower ? i - j : j - i;
-        if (dist <= done && dist <= kChainCache)
-          return dist == 1 ? xc[0] : dist == 2 ? xc[1] : dist == 3 ? xc[2] : xc[3];
-        return ent ? x[6 * (size_t)j + c] : T(0);
-      };
+      // sum the slots of scalar row r into lane r
+      T rs = T(0);
 #pragma unroll
-      for (int q = 0; q < kPrefetchBlocks; q++)
-        if (q < nb) acc += pre[q] * xdep(q);
-      for (int q = kPrefetchBlocks; q < nb; q++)
-        acc += (ent ? lu[36 * (size_t)(k0 + q) + lane] : T(0)) * xdep(q);
-      // reduce over c into lane r (lanes 0..5 then hold row sums)
-      T rs = 0;
-#pragma unroll
-      for (int m = 0; m < 6; m++) rs += __shfl(acc, 6 * (lane % 6) + m, 64);
+      for (int u = 0; u < kSlots; u++) rs += __shfl(acc, 6 * u + (lane % 6), 64);
       s -= (lane < 6) ? rs : T(0);
-      // 6x6 diagonal block: unit lower (L) or upper with diagonal (U)
       if (kLower) {
 #pragma unroll
         for (int m = 0; m < 5; m++) {
@@ -301,15 +373,10 @@ __global__ void __launch_bounds__(kSweepThreads)
           if (lane < 6 && lane < m) s -= drow[m] * ym;
         }
       }
-      if (lane < 6) x[6 * (size_t)i + lane] = s;
-      // shift the register cache: xc[0] = x[i][c]
+      if (lane < 6) store_value(x + 6 * (size_t)i + lane, s);
 #pragma unroll
-      for (int d = kChainCache - 1; d > 0; d--) xc[d] = xc[d - 1];
-      xc[0] = __shfl(s, c, 64);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (lane == 0)
-        __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      for (int c = 0; c < 6; c++) xprev[c] = __shfl(s, c, 64);
+      prev = i;
     }
   }
 }
@@ -348,24 +415,23 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
   std::vector<int> lrows, urows;
   gf::level_schedule(n, rowp, cols, true, h->lptr, lrows);
   gf::level_schedule(n, rowp, cols, false, h->uptr, urows);
-  // chains: a row continues the previous row's chain (in sweep order) when it
-  // depends on that row
-  std::vector<int> lch{0}, uch{0};
-  for (int i = 1; i < n; i++) {
-    bool dep = false;
-    for (int k = rowp[i]; k < diag[i]; k++) dep |= cols[k] == i - 1;
-    if (!dep) lch.push_back(i);
-  }
-  lch.push_back(n);
-  for (int t = 1; t < n; t++) {
-    const int i = n - 1 - t;
-    bool dep = false;
-    for (int k = diag[i] + 1; k < rowp[i + 1]; k++) dep |= cols[k] == i + 1;
-    if (!dep) uch.push_back(t);
-  }
-  uch.push_back(n);
-  h->nlchain = (int)lch.size() - 1;
-  h->nuchain = (int)uch.size() - 1;
+  // chunks: runs of rows of one level, sized so that a level still spreads
+  // over the resident wavefronts (one row per chunk for small levels)
+  const int target_waves = 16 * std::max(ctx->num_cus, 1);
+  auto chunks = [&](const std::vector<int>& lp, std::vector<int>& out) {
+    out.assign(1, 0);
+    for (size_t l = 0; l + 1 < lp.size(); l++) {
+      const int cnt = lp[l + 1] - lp[l];
+      const int csz = std::max(1, std::min(16, cnt / target_waves));
+      for (int a = lp[l]; a < lp[l + 1]; a += csz)
+        out.push_back(std::min(a + csz, lp[l + 1]));
+    }
+  };
+  std::vector<int> lch, uch;
+  chunks(h->lptr, lch);
+  chunks(h->uptr, uch);
+  h->nlchunk = (int)lch.size() - 1;
+  h->nuchunk = (int)uch.size() - 1;
   auto up = [&](int** dst, const int* src, size_t cnt) -> int {
     GF_HIP(ctx, hipMalloc(dst, sizeof(int) * (cnt ? cnt : 1)));
     GF_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(int) * cnt,
@@ -377,15 +443,12 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
       (st = up(&h->diag, diag.data(), n)) ||
       (st = up(&h->lrows, lrows.data(), n)) ||
       (st = up(&h->urows, urows.data(), n)) ||
-      (st = up(&h->lchain, lch.data(), lch.size())) ||
-      (st = up(&h->uchain, uch.data(), uch.size())) ||
+      (st = up(&h->lchunk, lch.data(), lch.size())) ||
+      (st = up(&h->uchunk, uch.data(), uch.size())) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->pivot, sizeof(int)), "malloc")) ||
-      (st = gf::check_hip(ctx, hipMalloc(&h->flags, sizeof(int) * n), "malloc")) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->counter, sizeof(unsigned long long)),
                           "malloc")) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->err, sizeof(int)), "malloc")) ||
-      (st = gf::check_hip(ctx, hipMemsetAsync(h->flags, 0, sizeof(int) * n, ctx->stream),
-                          "memset")) ||
       (st = gf::check_hip(ctx, hipMemsetAsync(h->counter, 0, sizeof(unsigned long long),
                                               ctx->stream), "memset")) ||
       (st = gf::check_hip(ctx, hipMemsetAsync(h->err, 0, sizeof(int), ctx->stream),
@@ -394,9 +457,9 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
     return st;
   }
   {
-    // enough resident wavefronts to cover the device, capped by the chains
-    const int waves = std::max(1, std::min(std::max(h->nlchain, h->nuchain),
-                                           16 * std::max(ctx->num_cus, 1)));
+    // enough wavefronts to cover the device, capped by the chunks
+    const int waves = std::max(1, std::min(std::max(h->nlchunk, h->nuchunk),
+                                           target_waves));
     h->grid = (waves + 3) / 4;
     const char* env = getenv("GPUFEM_TRSV");
     h->level_sweeps = env && std::string(env) == "level";
@@ -418,9 +481,8 @@ int gf_ilu_destroy(gf_ilu h) {
   hipFree(h->lrows);
   hipFree(h->urows);
   hipFree(h->pivot);
-  hipFree(h->flags);
-  hipFree(h->lchain);
-  hipFree(h->uchain);
+  hipFree(h->lchunk);
+  hipFree(h->uchunk);
   hipFree(h->counter);
   hipFree(h->err);
   delete h;
@@ -430,9 +492,9 @@ int gf_ilu_destroy(gf_ilu h) {
 int gf_ilu_info(gf_ilu h, int* nnzb, int* nl, int* nu) {
   if (!h) return GF_ERR_INVALID;
   if (getenv("GPUFEM_ILU_VERBOSE"))
-    fprintf(stderr, "ilu: n=%d nnzb=%d levels L=%d U=%d chains L=%d U=%d\n",
+    fprintf(stderr, "ilu: n=%d nnzb=%d levels L=%d U=%d chunks L=%d U=%d\n",
             h->n, h->nnzb, (int)h->lptr.size() - 1, (int)h->uptr.size() - 1,
-            h->nlchain, h->nuchain);
+            h->nlchunk, h->nuchunk);
   if (nnzb) *nnzb = h->nnzb;
   if (nl) *nl = (int)h->lptr.size() - 1;
   if (nu) *nu = (int)h->uptr.size() - 1;
@@ -492,16 +554,19 @@ int ilu_apply(gf_ilu h, const T* lu, const T* b, T* tmp, T* x) {
   gf_context ctx = h->ctx;
   if (!h->level_sweeps) {
     const unsigned long long waves = 4ull * h->grid;
+    const size_t N = 6 * (size_t)h->n;
+    hipLaunchKernelGGL(gf::fill_sentinel_kernel<T>, dim3(gf::div_up(N, 256)),
+                       dim3(256), 0, ctx->stream, N, tmp);
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
-                       h->cols, h->diag, h->lchain, h->nlchain, lu, b, tmp,
-                       h->flags, ++h->epoch, h->counter, h->base, h->err);
-    h->base += (unsigned long long)h->nlchain + waves;
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->rowp,
+                       h->cols, h->diag, h->lrows, h->lchunk, h->nlchunk, lu,
+                       b, tmp, x, h->counter, h->base, h->err);
+    h->base += (unsigned long long)h->nlchunk + waves;
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
-                       h->cols, h->diag, h->uchain, h->nuchain, lu, tmp, x,
-                       h->flags, ++h->epoch, h->counter, h->base, h->err);
-    h->base += (unsigned long long)h->nuchain + waves;
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->rowp,
+                       h->cols, h->diag, h->urows, h->uchunk, h->nuchunk, lu,
+                       tmp, x, (T*)nullptr, h->counter, h->base, h->err);
+    h->base += (unsigned long long)h->nuchunk + waves;
     return gf::check_launch(ctx, "trsv_syncfree_kernel");
   }
   const int nl = (int)h->lptr.size() - 1;

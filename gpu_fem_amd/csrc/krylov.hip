@@ -57,20 +57,30 @@ __global__ void __launch_bounds__(kDotThreads)
   }
 }
 
-// out[i] (+)= sum_b partial[b*nvec+i]; mode 0: store, 1: add, 2: store sqrt
-__global__ void multi_dot_final(int nblocks, int nvec,
-                                const double* __restrict__ partial,
-                                double* __restrict__ out, int mode) {
-  const int i = threadIdx.x;
-  if (i >= nvec) return;
+// out[i] (+)= sum_b partial[b*nvec+i]; mode 0: store, 1: add, 2: store sqrt.
+// One workgroup per i; fixed summation tree (deterministic).
+__global__ void __launch_bounds__(kDotThreads)
+    multi_dot_final(int nblocks, int nvec, const double* __restrict__ partial,
+                    double* __restrict__ out, int mode) {
+  __shared__ double red[kDotThreads / 64];
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   double s = 0.0;
-  for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * nvec + i];
-  if (mode == 1)
-    out[i] += s;
-  else if (mode == 2)
-    out[i] = sqrt(s);
-  else
-    out[i] = s;
+  for (int b = threadIdx.x; b < nblocks; b += kDotThreads)
+    s += partial[(size_t)b * nvec + i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < kDotThreads / 64; w++) v += red[w];
+    if (mode == 1)
+      out[i] += v;
+    else if (mode == 2)
+      out[i] = sqrt(v);
+    else
+      out[i] = v;
+  }
 }
 
 // w -= sum_i h[i] V_i
@@ -237,7 +247,7 @@ int KrylovWork::dots(const double* Vb, size_t ldv, int nvec, const double* x,
                      double* out, int mode) {
   hipLaunchKernelGGL(multi_dot_partial, dim3(kDotBlocks), dim3(kDotThreads), 0,
                      ctx->stream, N, nvec, Vb, ldv, x, partial);
-  hipLaunchKernelGGL(multi_dot_final, dim3(1), dim3(64 * div_up(nvec, 64)), 0,
+  hipLaunchKernelGGL(multi_dot_final, dim3(nvec), dim3(kDotThreads), 0,
                      ctx->stream, (int)kDotBlocks, nvec, partial, out, mode);
   return check_launch(ctx, "multi_dot");
 }

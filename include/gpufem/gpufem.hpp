@@ -542,6 +542,18 @@ void direct_LU_solve(BsrMat<Vec>& mat, Vec& rhs, Vec& soln) {
   check(gf_ilu_check(ilu.get()), "LU solve");
   bsr_post_solve(mat, rhs, soln);
 }
+// max-magnitude entry (signed) of K u - f, as CUSPARSE::get_resid
+// (include/linalg/solvers/cusparse.h:853-918); rhs and soln in the matrix
+// numbering
+template <class Vec>
+typename Vec::type get_resid(BsrMat<Vec> mat, Vec rhs, Vec soln) {
+  const BsrData d = mat.getBsrData();
+  double mx = 0.0, nrm = 0.0;
+  check(gf_bsr_resid_f64(context(), d.nnodes, d.rowPtr, d.colPtr, mat.getPtr(),
+                         soln.getPtr(), rhs.getPtr(), &mx, &nrm),
+        "get_resid");
+  return mx;
+}
 }  // namespace solvers
 
 // ------------------------------------------------------- shell type tags

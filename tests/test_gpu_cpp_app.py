@@ -53,6 +53,8 @@ def test_gmres_flow_matches_oracle(rcm, k):
     assert st == 0
     xo = oracle.permute_vec(np.argsort(s["perm"]).astype(np.int32), xp)
     assert info["nnzb_K"] == s["krp"][-1] and info["nnzb_ILU"] == s["mrp"][-1]
+    # get_resid: largest entry of K u - f, bounded by the 2-norm criterion
+    assert abs(info["max_resid"]) <= 1e-10 * np.linalg.norm(b) * 1.0001
     assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
 
 

@@ -269,3 +269,35 @@ def test_syncfree_sweeps_match_level_schedule(gpu_ctx, dtype, monkeypatch):
     x_o = oracle.bsr_trsv(s["mrp"], s["mcp"], lu, b.astype(np.float64))
     if dtype == np.float64:
         assert np.abs(outs[1] - x_o).max() <= 1e-10 * np.abs(x_o).max()
+
+
+def test_residual_diagnostic(gpu_ctx):
+    """gf_bsr_resid_f64 (CUSPARSE::get_resid, cusparse.h:853-918): the signed
+    entry of A x - b of largest magnitude, first index on ties, and the
+    2-norm."""
+    import ctypes
+
+    say("[gpu] bsr residual diagnostic plate 12x9 ss")
+    m = M.plate(12, 9, bc="ss", load="sine")
+    s = oracle_problem(m, "rcm", 0)
+    n = m.nnodes
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(6 * n)
+    b = rng.standard_normal(6 * n)
+    d = {k: dev(s[k]) for k in ("krp", "kcp", "kv")}
+    L = G.lib()
+    mx, nr = ctypes.c_double(), ctypes.c_double()
+    for xv, bv in ((x, b), (np.zeros(6 * n), None)):
+        if bv is None:  # planted tie: |r| maximal at 5 and 17, opposite signs
+            bv = rng.uniform(-1.0, 1.0, 6 * n)
+            bv[5], bv[17] = -3.0, 3.0
+        d_x, d_b = dev(xv), dev(bv)
+        gpu_ctx.check(L.gf_bsr_resid_f64(gpu_ctx.h, n, G.ptr(d["krp"]),
+                                         G.ptr(d["kcp"]), G.ptr(d["kv"]),
+                                         G.ptr(d_x), G.ptr(d_b),
+                                         ctypes.byref(mx), ctypes.byref(nr)))
+        r = oracle.bsr_spmv(s["krp"], s["kcp"], s["kv"], xv) - bv
+        i = int(np.argmax(np.abs(r)))
+        assert abs(mx.value - r[i]) <= 1e-12 * abs(r[i])
+        assert abs(nr.value - np.linalg.norm(r)) <= 1e-12 * np.linalg.norm(r)
+    assert mx.value == 3.0  # r = -b: first of the tied entries is r[5] = 3
