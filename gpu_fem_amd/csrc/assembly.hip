@@ -11,6 +11,9 @@
 //   stage 4  all 64 lanes: a 3x3 tile of K_e = sum_q B_q^T D_q (8x8 tiles)
 //   stage 5  scatter through elemIndMap (global FP64 atomics), residual
 //            R_e = K_e u_e reduced through LDS.
+#include <cstdlib>
+#include <string>
+
 #include "common.h"
 #include "mitc4.h"
 
@@ -156,6 +159,168 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched Jacobian: six elements per 64-lane wavefront, ten lanes per
+// element.  Lane l of an element owns one of the ten node-pair blocks
+// (a <= b) of the symmetric K_e and keeps its 6x6 sum in registers; the
+// quadrature points are streamed: per point the element's ten lanes form the
+// 24 columns of B_q and D_q = detXd C B_q into LDS (two or three columns per
+// lane) and every lane adds B_q[:,a]^T D_q[:,b] to its block.  Block (b, a)
+// is the transpose of (a, b).  LDS per element: geometry, one quadrature
+// point's B and D, the element residual.
+constexpr int kElemsPerWave = 6;
+constexpr int kLanesPerElem = 10;
+
+struct ElemSlot {
+  double X[12];
+  double u[24];
+  double data[4];
+  mitc4::Geom g;
+  double B[9][24];
+  double D[9][24];
+  double res[24];
+};
+
+// block index l (0..9) -> (a, b), a <= b
+__device__ __forceinline__ void pair_of(int l, int& a, int& b) {
+  a = l < 4 ? 0 : l < 7 ? 1 : l < 9 ? 2 : 3;
+  b = a + (l - (a == 0 ? 0 : a == 1 ? 4 : a == 2 ? 7 : 9));
+}
+
+__global__ void __launch_bounds__(64)
+    jacobian_batched_kernel(int nelems, const int* __restrict__ conn,
+                            const double* __restrict__ xpts,
+                            const double* __restrict__ vars,
+                            const double* __restrict__ data,
+                            const int* __restrict__ elem_ind_map,
+                            const int* __restrict__ perm, double* kvals,
+                            double* res) {
+  __shared__ ElemSlot S[kElemsPerWave];
+  const int lane = threadIdx.x;
+  const int es = lane / kLanesPerElem;
+  const int l = lane % kLanesPerElem;
+  const int e = blockIdx.x * kElemsPerWave + es;
+  const bool valid = es < kElemsPerWave && e < nelems;
+  ElemSlot& s = S[es < kElemsPerWave ? es : 0];
+  if (valid) {
+    for (int v = l; v < 40; v += kLanesPerElem) {
+      if (v < 12) {
+        s.X[v] = xpts[3 * (size_t)conn[4 * (size_t)e + v / 3] + v % 3];
+      } else if (v < 36) {
+        const int i = v - 12;
+        s.u[i] = vars ? vars[6 * (size_t)conn[4 * (size_t)e + i / 6] + i % 6]
+                      : 0.0;
+      } else {
+        s.data[v - 36] = data[4 * (size_t)e + (v - 36)];
+      }
+    }
+    for (int v = l; v < 24; v += kLanesPerElem) s.res[v] = 0.0;
+  }
+  __syncthreads();
+  if (valid) {
+    if (l < 4)
+      mitc4::geom_node(l, s.X, s.g);
+    else if (l < 9)
+      mitc4::geom_tying_x(l - 4, s.X, s.g);
+  }
+  __syncthreads();
+  if (valid) {
+    if (l < 4)
+      mitc4::geom_quad(l, s.X, s.g);
+    else if (l < 9)
+      mitc4::geom_tying_n(l - 4, s.g);
+  }
+  __syncthreads();
+  int a, b;
+  pair_of(l, a, b);
+  double acc[6][6];
+#pragma unroll
+  for (int r = 0; r < 6; r++)
+#pragma unroll
+    for (int c = 0; c < 6; c++) acc[r][c] = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < 4; q++) {
+    if (valid) {
+#pragma unroll 1
+      for (int j = l; j < 24; j += kLanesPerElem) {
+        mitc4::BColumnPrep prep;
+        mitc4::b_column_prep(j, s.g, prep);
+        double col[9], st[9];
+        mitc4::b_column_q(j, q, s.g, prep, col);
+        mitc4::stress(s.data, col, st);
+        const double sc = s.g.scale[q];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          s.B[k][j] = col[k];
+          s.D[k][j] = sc * st[k];
+        }
+      }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        double ba[6], db[6];
+#pragma unroll
+        for (int r = 0; r < 6; r++) ba[r] = s.B[k][6 * a + r];
+#pragma unroll
+        for (int c = 0; c < 6; c++) db[c] = s.D[k][6 * b + c];
+#pragma unroll
+        for (int r = 0; r < 6; r++)
+#pragma unroll
+          for (int c = 0; c < 6; c++) acc[r][c] += ba[r] * db[c];
+      }
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    const size_t blk = (size_t)elem_ind_map[16 * (size_t)e + 4 * a + b];
+    double* dst = kvals + 36 * blk;
+#pragma unroll
+    for (int r = 0; r < 6; r++)
+#pragma unroll
+      for (int c = 0; c < 6; c++) unsafeAtomicAdd(&dst[6 * r + c], acc[r][c]);
+    if (a != b) {
+      const size_t blt = (size_t)elem_ind_map[16 * (size_t)e + 4 * b + a];
+      double* dt = kvals + 36 * blt;
+#pragma unroll
+      for (int r = 0; r < 6; r++)
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+          unsafeAtomicAdd(&dt[6 * c + r], acc[r][c]);
+    }
+  }
+  if (!res) return;  // uniform across the workgroup
+  if (valid) {
+    // R_e = K_e u_e: rows of node a from block (a, b), rows of node b from
+    // its transpose
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      double v = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; c++) v += acc[r][c] * s.u[6 * b + c];
+      atomicAdd(&s.res[6 * a + r], v);
+    }
+    if (a != b) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        double v = 0.0;
+#pragma unroll
+        for (int r = 0; r < 6; r++) v += acc[r][c] * s.u[6 * a + r];
+        atomicAdd(&s.res[6 * b + c], v);
+      }
+    }
+  }
+  __syncthreads();
+  if (valid) {
+    for (int v = l; v < 24; v += kLanesPerElem) {
+      const int node = conn[4 * (size_t)e + v / 6];
+      const int row = perm ? perm[node] : node;
+      unsafeAtomicAdd(&res[6 * (size_t)row + v % 6], s.res[v]);
+    }
+  }
+}
+
 // Residual only: R_e = sum_q D_q^T (B_q u) (C symmetric).
 __global__ void __launch_bounds__(64)
     residual_kernel(int nelems, const int* __restrict__ conn,
@@ -235,10 +400,21 @@ int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
       !elem_ind_map || !kvals)
     return gf::set_error(ctx, GF_ERR_INVALID, "assemble_jacobian: bad args");
   if (nelems == 0) return GF_OK;
-  hipLaunchKernelGGL(gf::jacobian_atomic_kernel, dim3(nelems), dim3(64), 0,
-                     ctx->stream, nelems, conn, xpts, vars, data,
-                     elem_ind_map, perm, kvals, res);
-  return gf::check_launch(ctx, "jacobian_atomic_kernel");
+  static const bool per_element = [] {
+    const char* v = getenv("GPUFEM_ASSEMBLY");
+    return v && std::string(v) == "element";
+  }();
+  if (per_element) {
+    hipLaunchKernelGGL(gf::jacobian_atomic_kernel, dim3(nelems), dim3(64), 0,
+                       ctx->stream, nelems, conn, xpts, vars, data,
+                       elem_ind_map, perm, kvals, res);
+    return gf::check_launch(ctx, "jacobian_atomic_kernel");
+  }
+  hipLaunchKernelGGL(gf::jacobian_batched_kernel,
+                     dim3(gf::div_up(nelems, gf::kElemsPerWave)), dim3(64), 0,
+                     ctx->stream, nelems, conn, xpts, vars, data, elem_ind_map,
+                     perm, kvals, res);
+  return gf::check_launch(ctx, "jacobian_batched_kernel");
 }
 
 int gf_assemble_residual_mitc4_f64(gf_context ctx, int nelems, int nnodes,
