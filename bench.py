@@ -40,6 +40,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# precision modes of gf_options.precision
+PREC_NAMES = {0: "fp64", 1: "mixed", 2: "mixed_pc"}
 
 
 def parse():
@@ -48,7 +50,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nx", type=int, default=400)
-    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--precision", default="all",
+                    choices=["all", "both", "fp64", "mixed", "mixed_pc"])
     ap.add_argument("--reorder", default="natural", choices=["natural", "rcm", "amd"])
     ap.add_argument("--lev-fill", type=int, default=3)
     ap.add_argument("--restart", type=int, default=150)
@@ -125,11 +128,12 @@ def main():
     stream = ctx.stream
     d_loads = torch.from_numpy(mesh.loads.copy()).cuda()
     d_x = torch.zeros_like(d_loads)
-    precs = {"both": [1, 0], "fp64": [0], "mixed": [1]}[args.precision]
+    precs = {"all": [2, 1, 0], "both": [1, 0], "fp64": [0], "mixed": [1],
+             "mixed_pc": [2]}[args.precision]
     results = {}
     problems = {}
     for prec in precs:
-        name = "mixed" if prec else "fp64"
+        name = PREC_NAMES[prec]
         t0 = time.time()
         reorder_id = {"natural": 0, "rcm": 1, "amd": 2}[args.reorder]
         p = G.Problem(ctx, mesh, reorder=reorder_id, lev_fill=args.lev_fill,
@@ -169,14 +173,15 @@ def main():
             t_factor_ms=info.t_factor * 1e3, t_solve_ms=info.t_solve * 1e3,
             x_host=d_x.cpu().numpy().copy())
 
-    # mixed vs FP64 solution difference (error of the mixed result vs FP64)
-    if "mixed" in results and "fp64" in results:
-        a, b = results["mixed"]["x_host"], results["fp64"]["x_host"]
-        results["mixed"]["rel_err_vs_fp64"] = float(
-            np.linalg.norm(a - b) / np.linalg.norm(b))
+    # mixed vs FP64 solution difference (error of the mixed results vs FP64)
+    for mname in ("mixed", "mixed_pc"):
+        if mname in results and "fp64" in results:
+            a, b = results[mname]["x_host"], results["fp64"]["x_host"]
+            results[mname]["rel_err_vs_fp64"] = float(
+                np.linalg.norm(a - b) / np.linalg.norm(b))
 
     # ---- dominant kernel: the triangular sweeps; time them in isolation
-    pname = "fp64" if "fp64" in problems else "mixed"
+    pname = "fp64" if "fp64" in problems else next(iter(problems))
     p = problems[pname]
     d = p.device_arrays()
     nn, nk, nm = p.sizes()
@@ -244,7 +249,7 @@ def main():
                            results[pname]["iterations"], args.cpu_iters)
 
     best = None
-    for name in ("mixed", "fp64"):
+    for name in ("mixed_pc", "mixed", "fp64"):
         if name in results and results[name]["converged"]:
             if best is None or results[name]["solves_per_s"] > results[best]["solves_per_s"]:
                 best = name
@@ -264,6 +269,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if best == "fp64" else "f64/f32",
+        "precision_modes": {"fp64": "FP64 ILU, sweeps, SpMV, Arnoldi",
+                            "mixed": "GMRES-IR: FP32 ILU, sweeps and inner "
+                                     "SpMV; FP64 residual, Arnoldi, update",
+                            "mixed_pc": "FP32 ILU and sweeps inside FP64 "
+                                        "GMRES (FP64 SpMV, Arnoldi)"},
         "data": "synthetic",
         "config": {"workload": f"plate {nx}x{nx} MITC4, t=5mm, SS, sine "
                                f"pressure; {args.reorder} order + "

@@ -167,9 +167,10 @@ int problem_setup(gf_problem p, const int* conn, const double* xpts,
   GF_TRY(dalloc(ctx, &p->loads, N));
   GF_TRY(dalloc(ctx, &p->work64, N));
   GF_TRY(dalloc(ctx, &p->work64b, N));
-  const bool mixed = p->opt.precision == 1;
+  const bool mixed = p->opt.precision != 0;
   if (mixed) {
-    GF_TRY(dalloc(ctx, &p->kvals32, 36 * (size_t)p->nnzb_k));
+    if (p->opt.precision == 1)
+      GF_TRY(dalloc(ctx, &p->kvals32, 36 * (size_t)p->nnzb_k));
     GF_TRY(dalloc(ctx, &p->mvals32, 36 * (size_t)p->nnzb_m));
     GF_TRY(dalloc(ctx, &p->t32a, N));
     GF_TRY(dalloc(ctx, &p->t32b, N));
@@ -218,7 +219,7 @@ int gf_problem_create(gf_context ctx, int nnodes, int nelems, const int* conn,
     gf_options_default(&p->opt);
   if (p->opt.restart < 1 || p->opt.restart > 512 || p->opt.lev_fill < 0 ||
       p->opt.reorder < 0 || p->opt.reorder > 2 || p->opt.precision < 0 ||
-      p->opt.precision > 1) {
+      p->opt.precision > 2) {
     delete p;
     return gf::set_error(ctx, GF_ERR_INVALID, "problem_create: bad options");
   }
@@ -259,11 +260,12 @@ int gf_problem_factor(gf_problem p) {
   gf_context ctx = p->ctx;
   GF_HIP(ctx, hipEventRecord(p->ev[0], ctx->stream));
   int zp = -1;
-  if (p->opt.precision == 1) {
+  if (p->opt.precision != 0) {
     GF_TRY(gf_convert_f64_f32(ctx, 36 * (size_t)p->nnzb_m, p->mvals,
                               p->mvals32));
-    GF_TRY(gf_convert_f64_f32(ctx, 36 * (size_t)p->nnzb_k, p->kvals,
-                              p->kvals32));
+    if (p->opt.precision == 1)
+      GF_TRY(gf_convert_f64_f32(ctx, 36 * (size_t)p->nnzb_k, p->kvals,
+                                p->kvals32));
     GF_TRY(gf_bsr_ilu_factor_f32(p->ilu, p->mvals32, &zp));
   } else {
     GF_TRY(gf_bsr_ilu_factor_f64(p->ilu, p->mvals, &zp));
@@ -305,7 +307,7 @@ int gf_problem_solve(gf_problem p, const double* loads, double* x,
   op.t32c = p->t32c;
   gf::GmresResult gr;
   const int st = gf::gmres_solve(p->kw, op, p->rhs, p->sol, p->opt.max_iters,
-                                 p->opt.rtol, p->opt.precision == 1, &gr);
+                                 p->opt.rtol, p->opt.precision, &gr);
   // x = inverse permutation of sol: x[old] = sol[perm[old]]
   GF_TRY(gf_permute_vec(ctx, p->nnodes, p->iperm, p->sol, p->work64));
   GF_HIP(ctx, hipMemcpyAsync(x, p->work64, sizeof(double) * N,
@@ -364,10 +366,11 @@ extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
                               const double* b, double* x, int restart,
                               int max_iters, double rtol, int precision,
                               int* iterations, double* rel_resid) {
-  const bool mixed = precision == 1;
+  const bool mixed = precision != 0;
   if (!ctx || nnodes <= 0 || !krowp || !kcols || !kvals || !ilu || !b || !x ||
-      restart < 1 || restart > 512 || precision < 0 || precision > 1 ||
-      (!mixed && !mlu) || (mixed && (!mlu32 || !kvals32)))
+      restart < 1 || restart > 512 || precision < 0 || precision > 2 ||
+      (!mixed && !mlu) || (mixed && !mlu32) ||
+      (precision == 1 && !kvals32))
     return gf::set_error(ctx, GF_ERR_INVALID, "gmres_solve: bad args");
   const size_t N = 6 * (size_t)nnodes;
   gf::KrylovWork kw;
@@ -398,7 +401,7 @@ extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
     op.t32a = a;
     op.t32b = bb;
     op.t32c = c;
-    st = gf::gmres_solve(kw, op, b, x, max_iters, rtol, mixed, &gr);
+    st = gf::gmres_solve(kw, op, b, x, max_iters, rtol, precision, &gr);
   }
   hipStreamSynchronize(ctx->stream);
   hipFree(w64);

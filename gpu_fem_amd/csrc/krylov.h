@@ -37,8 +37,10 @@ struct Operator {
   double *work64 = nullptr, *work64b = nullptr;
   float *t32a = nullptr, *t32b = nullptr, *t32c = nullptr;
   int apply_a64(const double* in, double* out) const;
-  int apply_prec(const double* in, double* out, bool mixed) const;
-  int apply_prec_a(const double* in, double* out, bool mixed) const;
+  // prec: 0 FP64 ILU; 1 FP32 ILU and FP32 SpMV (inner loop of GMRES-IR);
+  // 2 FP32 ILU, FP64 SpMV
+  int apply_prec(const double* in, double* out, int prec) const;
+  int apply_prec_a(const double* in, double* out, int prec) const;
 };
 
 struct GmresResult {
@@ -47,7 +49,9 @@ struct GmresResult {
   double rel_resid = 0.0;
 };
 
+// prec as in Operator: 1 runs GMRES-IR (inner cycles stop at a 1e-5
+// reduction of the FP64 residual), 0 and 2 run plain restarted GMRES.
 int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
-                int max_iters, double rtol, bool mixed, GmresResult* res);
+                int max_iters, double rtol, int prec, GmresResult* res);
 
 }  // namespace gf

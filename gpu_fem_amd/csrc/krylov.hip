@@ -253,7 +253,8 @@ int KrylovWork::dots(const double* Vb, size_t ldv, int nvec, const double* x,
 }
 
 int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
-                int max_iters, double rtol, bool mixed, GmresResult* res) {
+                int max_iters, double rtol, int prec, GmresResult* res) {
+  const bool ir = prec == 1;
   gf_context ctx = W.ctx;
   hipStream_t s = ctx->stream;
   const size_t N = W.N;
@@ -288,7 +289,7 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     if (it >= max_iters) break;
     outer++;
     const double tol_abs =
-        mixed ? std::max(1e-5 * beta, 0.5 * rtol * bnorm) : rtol * bnorm;
+        ir ? std::max(1e-5 * beta, 0.5 * rtol * bnorm) : rtol * bnorm;
     hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.r, W.nrm,
                        W.V);
     hipLaunchKernelGGL(init_g, dim3(1), dim3(64 * div_up(m + 1, 64)), 0, s, m,
@@ -296,7 +297,7 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     int j = 0;
     while (j < m && it < max_iters) {
       double* vj = W.V + (size_t)j * N;
-      GF_TRY(op.apply_prec_a(vj, W.w, mixed));  // w = A M^-1 v_j
+      GF_TRY(op.apply_prec_a(vj, W.w, prec));  // w = A M^-1 v_j
       it++;
       // CGS2
       GF_TRY(W.dots(W.V, N, j + 1, W.w, W.h, 0));
@@ -326,7 +327,7 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
                        W.y);
     hipLaunchKernelGGL(lincomb, grid1(N), dim3(256), 0, s, N, j, W.V, N, W.y,
                        W.tmp);
-    GF_TRY(op.apply_prec(W.tmp, W.z, mixed));
+    GF_TRY(op.apply_prec(W.tmp, W.z, prec));
     hipLaunchKernelGGL(axpy_k, grid1(N), dim3(256), 0, s, N, 1.0, W.z, x);
     GF_TRY(check_launch(ctx, "gmres update"));
   }
@@ -350,23 +351,29 @@ int Operator::apply_a64(const double* in, double* out) const {
   return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, in, out);
 }
 
-int Operator::apply_prec(const double* in, double* out, bool mixed) const {
+int Operator::apply_prec(const double* in, double* out, int prec) const {
   const size_t N = 6 * (size_t)n;
-  if (!mixed) return gf_bsr_ilu_apply_f64(ilu, mlu, in, work64, out);
+  if (prec == 0) return gf_bsr_ilu_apply_f64(ilu, mlu, in, work64, out);
   hipLaunchKernelGGL(to_f32, grid1(N), dim3(256), 0, ctx->stream, N, in, t32a);
   GF_TRY(gf_bsr_ilu_apply_f32(ilu, mlu32, t32a, t32b, t32c));
   hipLaunchKernelGGL(to_f64, grid1(N), dim3(256), 0, ctx->stream, N, t32c, out);
-  return check_launch(ctx, "apply_prec mixed");
+  return check_launch(ctx, "apply_prec fp32");
 }
 
-int Operator::apply_prec_a(const double* in, double* out, bool mixed) const {
+int Operator::apply_prec_a(const double* in, double* out, int prec) const {
   const size_t N = 6 * (size_t)n;
-  if (!mixed) {
+  if (prec == 0) {
     GF_TRY(gf_bsr_ilu_apply_f64(ilu, mlu, in, work64, work64b));
     return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, work64b, out);
   }
   hipLaunchKernelGGL(to_f32, grid1(N), dim3(256), 0, ctx->stream, N, in, t32a);
   GF_TRY(gf_bsr_ilu_apply_f32(ilu, mlu32, t32a, t32b, t32c));
+  if (prec == 2) {
+    hipLaunchKernelGGL(to_f64, grid1(N), dim3(256), 0, ctx->stream, N, t32c,
+                       work64b);
+    GF_TRY(check_launch(ctx, "apply_prec_a fp32 ilu"));
+    return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, work64b, out);
+  }
   GF_TRY(gf_bsr_spmv_f32(ctx, n, krowp, kcols, kvals32, t32c, t32a));
   hipLaunchKernelGGL(to_f64, grid1(N), dim3(256), 0, ctx->stream, N, t32a, out);
   return check_launch(ctx, "apply_prec_a mixed");

@@ -169,8 +169,10 @@ int gf_bsr_ilu_apply_f32(gf_ilu ilu, const float* d_lu, const float* d_b,
  *      (examples/iterative/gmres.cu:472-771 with restarts and a relative
  *      tolerance).  precision 0: FP64 operator and ILU factors (d_mlu);
  *      precision 1: GMRES-IR, inner FP32 ILU factors (d_mlu32) and FP32 copy
- *      of the operator (d_kvals32), FP64 outer residual and Arnoldi.  x holds
- *      the initial guess on entry.  info may be NULL. */
+ *      of the operator (d_kvals32), FP64 outer residual and Arnoldi;
+ *      precision 2: FP32 ILU factors (d_mlu32) and FP32 sweeps inside FP64
+ *      GMRES (FP64 SpMV, d_kvals32 unused).  x holds the initial guess on
+ *      entry. */
 int gf_gmres_solve(gf_context ctx, int nnodes, const int* d_krowp,
                    const int* d_kcols, const double* d_kvals,
                    const float* d_kvals32, gf_ilu ilu, const double* d_mlu,
@@ -185,7 +187,8 @@ typedef struct {
   int restart;       /* GMRES(m) restart length */
   int max_iters;     /* total Krylov iterations */
   double rtol;       /* ||b - A x|| <= rtol ||b|| in FP64 */
-  int precision;     /* 0 FP64, 1 mixed (FP32 ILU/trsv/SpMV, FP64 outer) */
+  int precision;     /* 0 FP64; 1 GMRES-IR (FP32 ILU/trsv/SpMV inside,
+                        FP64 outer); 2 FP32 ILU/trsv inside FP64 GMRES */
 } gf_options;
 
 typedef struct {

@@ -61,7 +61,7 @@ CASES = [
 
 @pytest.mark.parametrize("name,build,reorder,k,restart,rtol", CASES,
                          ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("precision", [0, 1, 2], ids=["fp64", "mixed", "mixed_pc"])
 def test_solve_matches_oracle(gpu_ctx, name, build, reorder, k, restart, rtol,
                               precision):
     say(f"[gpu] solve {name} precision={precision}")
