@@ -58,6 +58,16 @@ def parse():
     ap.add_argument("--max-iters", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=10)
+    # served request mix (separate JSON line; see serve_main)
+    ap.add_argument("--serve", action="store_true")
+    ap.add_argument("--requests", type=int, default=32)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--serve-scale", type=float, default=1.0)
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="Poisson arrival rate (requests/s); 0: all at t=0")
+    ap.add_argument("--serve-precision", type=int, default=2)
+    ap.add_argument("--serve-lev-fill", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=0)
     return ap.parse_args()
 
 
@@ -111,6 +121,9 @@ def main():
               f"--gpus {args.gpus}); the multi-GPU path is design-only",
               file=sys.stderr)
         sys.exit(2)
+    if args.serve:
+        serve_main(args)
+        return
     import torch
 
     import gpu_fem_amd as G
@@ -296,6 +309,117 @@ def main():
                               "bytes_per_launch": asm_bytes,
                               "launch_us": asm_ms * 1e3},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def serve_main(args):
+    """Served request mix: seeded plates and cylinders of differing sizes
+    (gpu_fem_amd.meshes.served_requests), each a complete linear-static solve
+    to 1e-8 relative residual (symbolic phase, upload, assembly, ILU, GMRES).
+
+    Plain path: FP64, one request at a time on one stream.  Served path:
+    gf_server with ``--slots`` worker streams under a 200 GB admission
+    budget, mixed precision (``--serve-precision``), requests submitted at
+    Poisson arrival times (``--rate``; 0 submits all at once, which measures
+    capacity).  Each served result is compared with the plain FP64 result of
+    the same request."""
+    import torch
+
+    import gpu_fem_amd as G
+    from gpu_fem_amd import meshes as M
+
+    torch.cuda.init()
+    t0 = time.time()
+    reqs = M.served_requests(args.requests, seed=args.seed,
+                             scale=args.serve_scale)
+    print(f"[serve] {len(reqs)} requests generated in {time.time()-t0:.1f}s; "
+          f"nodes {min(m.nnodes for m in reqs)}..{max(m.nnodes for m in reqs)}",
+          file=sys.stderr, flush=True)
+    opts = dict(reorder=2, lev_fill=args.serve_lev_fill, restart=100,
+                rtol=1e-8, max_iters=20000)
+    ctx = G.Context(0)
+    # plain FP64, one at a time
+    plain_x, plain_lat, plain_ok = {}, [], 0
+    t0 = time.perf_counter()
+    for i, m in enumerate(reqs):
+        ts = time.perf_counter()
+        p = G.Problem(ctx, m, precision=0, **opts)
+        p.assemble()
+        p.factor()
+        x, info = p.solve(raise_on_error=False)
+        p.close()
+        dt = time.perf_counter() - ts
+        plain_lat.append(dt)
+        plain_ok += int(info.status == 0)
+        plain_x[i] = x
+        print(f"[serve] plain {i} {m.name}: {dt*1e3:.1f} ms iters="
+              f"{info.iterations} resid={info.rel_resid:.2e} status={info.status}",
+              file=sys.stderr, flush=True)
+    t_plain = time.perf_counter() - t0
+    ctx.close()
+    # served
+    rng = np.random.default_rng(args.seed + 1)
+    n = len(reqs)
+    arrivals = (np.cumsum(rng.exponential(1.0 / args.rate, n))
+                if args.rate > 0 else np.zeros(n))
+    srv = G.Server(budget_gb=200.0, slots=args.slots)
+    got = {}
+
+    def drain_ready():
+        while True:
+            r = srv.poll()
+            if r is None:
+                return
+            got[r[0]] = r
+
+    t0 = time.perf_counter()
+    for i, m in enumerate(reqs):
+        while time.perf_counter() - t0 < arrivals[i]:
+            drain_ready()
+            time.sleep(0.0005)
+        code = srv.submit(i, m, precision=args.serve_precision, **opts)
+        if code != 0:
+            raise RuntimeError(f"submit {i} refused: {code}")
+    srv.drain()
+    t_served = time.perf_counter() - t0
+    drain_ready()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([got[i][3] for i in sorted(got)])
+    ok = sum(int(got[i][2].status == 0) for i in got)
+    errs = []
+    for i in sorted(got):
+        xs, xp = got[i][1], plain_x[i]
+        errs.append(float(np.linalg.norm(xs - xp) / max(np.linalg.norm(xp), 1e-300)))
+    served_sps = len(got) / t_served
+    plain_sps = n / t_plain
+    line = {
+        "metric": "served linear-static solves/s, p50/p99 latency @1e-8 resid",
+        "value": served_sps,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "requests": n,
+        "higher_is_better": True,
+        "data": "synthetic",
+        "config": {"workload": "served request mix: plates n^2, n log-uniform "
+                               "[32, 256], and cylinders 64x128..192x384, "
+                               "t U[1,10] mm, p U[0.5,5] kPa, clamped/SS",
+                   "scale": args.serve_scale, "seed": args.seed,
+                   "slots": args.slots, "budget_gb": 200.0,
+                   "rate_per_s": args.rate, "order": "AMD",
+                   "lev_fill": args.serve_lev_fill,
+                   "precision": PREC_NAMES[args.serve_precision]},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "served_ok": ok,
+        "plain_fp64_one_at_a_time": {
+            "solves_per_s": plain_sps, "ok": plain_ok,
+            "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(plain_lat, 99) * 1e3)},
+        "speedup_vs_plain": served_sps / plain_sps,
+        "max_rel_diff_vs_plain_fp64": max(errs) if errs else None,
+        "peak_admitted_gb": st["peak_bytes"] / 1e9,
     }
     print(json.dumps(line), flush=True)
 

@@ -150,8 +150,9 @@ def plate(nxe, nye, Lx=1.0, Ly=1.0, E=70e9, nu=0.3, thick=0.01,
 
     bc: "clamped" (all 6 DOFs on every edge node) or "ss" (the reference's
     simply supported pattern).  load: "uniform" (consistent, magnitude
-    load_mag Pa), "sine" (consistent load_mag sin(pi x/Lx) sin(pi y/Ly)) or
-    "nodal_sine" (the reference's point-sampled nodal values).
+    load_mag Pa), "sine" (consistent load_mag sin(pi x/Lx) sin(pi y/Ly)),
+    "uniform_sine" (consistent load_mag (1 + sin(pi x/Lx) sin(pi y/Ly)) / 2)
+    or "nodal_sine" (the reference's point-sampled nodal values).
     """
     conn = plate_conn(nxe, nye, ccw=ccw)
     xpts = plate_xpts(nxe, nye, Lx, Ly)
@@ -164,6 +165,10 @@ def plate(nxe, nye, Lx=1.0, Ly=1.0, E=70e9, nu=0.3, thick=0.01,
         loads = consistent_pressure_loads(
             conn, xpts, lambda x: load_mag * np.sin(np.pi * x[:, 0] / Lx) *
             np.sin(np.pi * x[:, 1] / Ly))
+    elif load == "uniform_sine":
+        loads = consistent_pressure_loads(
+            conn, xpts, lambda x: 0.5 * load_mag * (
+                1.0 + np.sin(np.pi * x[:, 0] / Lx) * np.sin(np.pi * x[:, 1] / Ly)))
     elif load == "nodal_sine":
         loads = plate_loads_nodal_reference(nxe, nye, Lx, Ly, load_mag)
     else:
@@ -175,10 +180,11 @@ def plate(nxe, nye, Lx=1.0, Ly=1.0, E=70e9, nu=0.3, thick=0.01,
 
 
 def cylinder(nc, na, R=1.0, L=3.0, E=70e9, nu=0.3, t_range=(1e-3, 8e-3),
-             seed=0, pressure=50e3, axial_line_load=10e3):
+             seed=0, pressure=50e3, axial_line_load=10e3, bc="clamped"):
     """Open cylinder, nc circumferential x na axial quads (periodic in theta).
 
-    Nodes: nc around x (na + 1) along z.  The z = 0 end is clamped; internal
+    Nodes: nc around x (na + 1) along z.  The z = 0 end is clamped (bc
+    "clamped", all 6 DOFs) or pinned (bc "ss", translations only); internal
     pressure acts outward (consistent loads) and a uniform axial line load
     (N/m) acts on the z = L end ring.  Per-element thickness is uniform in
     t_range from ``seed`` (numpy PCG64).
@@ -196,7 +202,8 @@ def cylinder(nc, na, R=1.0, L=3.0, E=70e9, nu=0.3, t_range=(1e-3, 8e-3),
     thick = rng.uniform(t_range[0], t_range[1], size=len(conn))
     data = _data(len(conn), E, nu, 1.0)
     data[:, 2] = thick
-    bcs = (6 * np.arange(nc)[:, None] + np.arange(6)[None, :]).ravel()
+    ndof = 6 if bc == "clamped" else 3
+    bcs = (6 * np.arange(nc)[:, None] + np.arange(ndof)[None, :]).ravel()
     radial = lambda x: np.stack([x[:, 0], x[:, 1], 0 * x[:, 0]], 1) / R
     loads = consistent_pressure_loads(conn, xpts,
                                       lambda x: np.full(len(x), pressure),
@@ -206,4 +213,33 @@ def cylinder(nc, na, R=1.0, L=3.0, E=70e9, nu=0.3, t_range=(1e-3, 8e-3),
     loads[bcs] = 0.0
     return Mesh(conn, xpts, data, bcs.astype(np.int32), loads,
                 name=f"cylinder{nc}x{na}",
-                meta=dict(nc=nc, na=na, R=R, L=L, seed=seed))
+                meta=dict(nc=nc, na=na, R=R, L=L, seed=seed, bc=bc))
+
+
+def served_requests(n, seed=0, scale=1.0):
+    """The served request mix: seeded plates (n x n quads, n log-uniform in
+    [32, 256]) and cylinders (nc x na from 64 x 128 to 192 x 384,
+    log-uniform), with probability 1/2 each; thickness uniform in [1, 10] mm;
+    uniform-plus-sinusoidal pressure with magnitude uniform in [0.5, 5] kPa;
+    clamped or simply supported with probability 1/2.  ``scale`` < 1 shrinks
+    every mesh side (for quick runs)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        t = float(rng.uniform(1e-3, 10e-3))
+        p0 = float(rng.uniform(500.0, 5000.0))
+        bc = "clamped" if rng.random() < 0.5 else "ss"
+        if rng.random() < 0.5:
+            m = int(round(np.exp(rng.uniform(np.log(32), np.log(256))) * scale))
+            m = max(m, 4)
+            mesh = plate(m, m, 1.0, 1.0, thick=t, bc=bc, load="uniform_sine",
+                         load_mag=p0)
+        else:
+            f = np.exp(rng.uniform(np.log(1.0), np.log(3.0)))
+            nc = max(int(round(64 * f * scale)), 8)
+            na = max(int(round(128 * f * scale)), 4)
+            mesh = cylinder(nc, na, t_range=(t, t), seed=int(rng.integers(1 << 30)),
+                            pressure=p0, axial_line_load=0.0, bc=bc)
+        mesh.meta["request"] = i
+        out.append(mesh)
+    return out
