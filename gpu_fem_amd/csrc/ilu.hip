@@ -53,9 +53,11 @@ namespace {
 
 constexpr int kMaxRowBlocks = 1024;
 
-// One 256-thread workgroup per row of the current level.
+// One 64-lane workgroup (a single wavefront, so barriers are cheap) per row
+// of the current level.
+constexpr int kIluThreads = 64;
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kIluThreads)
     ilu_row_kernel(const int* __restrict__ level_rows, int nrows,
                    const int* __restrict__ rowp, const int* __restrict__ cols,
                    const int* __restrict__ diag, T* vals, int* pivot) {
@@ -531,7 +533,7 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   const int nl = (int)h->lptr.size() - 1;
   for (int l = 0; l < nl; l++) {
     const int cnt = h->lptr[l + 1] - h->lptr[l];
-    hipLaunchKernelGGL(gf::ilu_row_kernel<T>, dim3(cnt), dim3(256), 0,
+    hipLaunchKernelGGL(gf::ilu_row_kernel<T>, dim3(cnt), dim3(gf::kIluThreads), 0,
                        ctx->stream, h->lrows + h->lptr[l], cnt, h->rowp,
                        h->cols, h->diag, vals, h->pivot);
   }
