@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--serve-precision", type=int, default=2)
     ap.add_argument("--serve-lev-fill", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0)
+    # short kernel-only run for profilers (rocprofv3 --pmc / --kernel-trace)
+    ap.add_argument("--profile-kernels", action="store_true")
     return ap.parse_args()
 
 
@@ -123,6 +125,9 @@ def main():
         sys.exit(2)
     if args.serve:
         serve_main(args)
+        return
+    if args.profile_kernels:
+        profile_kernels_main(args)
         return
     import torch
 
@@ -311,6 +316,49 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def profile_kernels_main(args):
+    """A short run of the hot kernels for profilers: one Jacobian assembly,
+    one numeric factorisation, three L+U sweep pairs and three SpMVs of the
+    benchmark problem (FP64, --reorder/--lev-fill), and a device copy of
+    1 GiB (the calibration kernel for HBM byte counters)."""
+    import torch
+
+    import gpu_fem_amd as G
+    from gpu_fem_amd import meshes as M
+
+    torch.cuda.init()
+    nx = args.nx
+    mesh = M.plate(nx, nx, 1.0, 1.0, E=70e9, nu=0.3, thick=0.005, bc="ss",
+                   load="sine", load_mag=1e3)
+    ctx = G.Context(0)
+    reorder_id = {"natural": 0, "rcm": 1, "amd": 2}[args.reorder]
+    p = G.Problem(ctx, mesh, reorder=reorder_id, lev_fill=args.lev_fill,
+                  precision=0)
+    p.assemble()
+    p.factor()
+    d = p.device_arrays()
+    nn, nk, nm = p.sizes()
+    L = G.lib()
+    n6 = 6 * nn
+    b = torch.ones(n6, dtype=torch.float64, device="cuda")
+    tmp = torch.zeros_like(b)
+    x = torch.zeros_like(b)
+    ilu = p.ilu_handle()
+    for _ in range(3):
+        ctx.check(L.gf_bsr_ilu_apply_f64(ilu, d["mvals"], G.ptr(b), G.ptr(tmp),
+                                         G.ptr(x)))
+        ctx.check(L.gf_bsr_spmv_f64(ctx.h, nn, d["krowp"], d["kcols"],
+                                    d["kvals"], G.ptr(b), G.ptr(x)))
+    ctx.sync()
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_kernels": True, "nodes": nn, "nnzb_K": nk,
+                      "nnzb_ILU": nm, "copy_bytes": 2 * src.numel() * 8}),
+          flush=True)
 
 
 def serve_main(args):

@@ -105,6 +105,7 @@ _SIGS = {
     "gf_problem_sizes": [_P, _PI, _PI, _PI],
     "gf_problem_device_arrays": [_P] + [ctypes.POINTER(_P)] * 7,
     "gf_problem_host_perm": [_P, _P],
+    "gf_problem_ilu": [_P],
     "gf_server_create": [_I, ctypes.c_double, _I, ctypes.POINTER(_P)],
     "gf_server_submit": [_P, ctypes.c_longlong, _I, _I, _P, _P, _P, _I, _P,
                          _P, ctypes.POINTER(Options)],
@@ -118,6 +119,7 @@ _SIGS = {
 _RESTYPES = {"gf_context_last_error": ctypes.c_char_p,
              "gf_context_stream": _P, "gf_free": None,
              "gf_options_default": None, "gf_ilu_device_rowp": _P,
+             "gf_problem_ilu": _P,
              "gf_ilu_device_cols": _P}
 
 
@@ -362,6 +364,10 @@ class Problem:
             self.h, *[ctypes.byref(p) for p in ps]))
         return dict(zip(["krowp", "kcols", "kvals", "mrowp", "mcols", "mvals",
                          "perm"], [p.value for p in ps]))
+
+    def ilu_handle(self):
+        """The problem's gf_ilu (owned by the problem)."""
+        return lib().gf_problem_ilu(self.h)
 
     def perm(self):
         out = np.zeros(self.mesh.nnodes, np.int32)

@@ -351,6 +351,8 @@ int gf_problem_device_arrays(gf_problem p, const int** krowp,
   return GF_OK;
 }
 
+gf_ilu gf_problem_ilu(gf_problem p) { return p ? p->ilu : nullptr; }
+
 int gf_problem_host_perm(gf_problem p, int* perm) {
   if (!p || !perm) return GF_ERR_INVALID;
   std::memcpy(perm, p->h_perm.data(), sizeof(int) * p->nnodes);

@@ -219,6 +219,8 @@ int gf_problem_device_arrays(gf_problem p, const int** krowp,
                              const int** kcols, double** kvals,
                              const int** mrowp, const int** mcols,
                              double** mvals, const int** perm);
+/* the problem's ILU analysis handle (owned by the problem) */
+gf_ilu gf_problem_ilu(gf_problem p);
 int gf_problem_host_perm(gf_problem p, int* perm);
 
 /* ---- serving loop (no reference counterpart: the served workload's request
