@@ -16,7 +16,8 @@ struct KrylovWork {
   double *h = nullptr, *h2 = nullptr, *nrm = nullptr, *y = nullptr;
   double *state = nullptr, *partial = nullptr, *scal = nullptr;
   float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
-  double* host_scal = nullptr;  // pinned
+  double* host_scal = nullptr;  // pinned: two (estimate, h_{j+1,j}) slots
+  hipEvent_t ev[2] = {nullptr, nullptr};
   int init(gf_context c, size_t N, int m, bool mixed);
   void release();
   int dots(const double* V, size_t ldv, int nvec, const double* x, double* out,

@@ -219,6 +219,10 @@ int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
   }
   GF_TRY(check_hip(ctx, hipHostMalloc((void**)&host_scal, sizeof(double) * 4),
                    "hipHostMalloc"));
+  GF_TRY(check_hip(ctx, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming),
+                   "event"));
+  GF_TRY(check_hip(ctx, hipEventCreateWithFlags(&ev[1], hipEventDisableTiming),
+                   "event"));
   return GF_OK;
 }
 
@@ -239,6 +243,8 @@ void KrylovWork::release() {
   hipFree(b32);
   hipFree(c32);
   if (host_scal) hipHostFree(host_scal);
+  if (ev[0]) hipEventDestroy(ev[0]);
+  if (ev[1]) hipEventDestroy(ev[1]);
   *this = KrylovWork();
 }
 
@@ -297,34 +303,52 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
                        W.V);
     hipLaunchKernelGGL(init_g, dim3(1), dim3(64 * div_up(m + 1, 64)), 0, s, m,
                        W.nrm, W.state);
-    int j = 0;
-    while (j < m && it < max_iters) {
-      double* vj = W.V + (size_t)j * N;
-      GF_TRY(op.apply_prec_a(vj, W.w, prec));  // w = A M^-1 v_j
-      it++;
-      // CGS2
-      GF_TRY(W.dots(W.V, N, j + 1, W.w, W.h, 0));
-      hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, j + 1, W.V, N,
-                         W.h, W.w);
-      GF_TRY(W.dots(W.V, N, j + 1, W.w, W.h2, 0));
-      hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, j + 1, W.V, N,
-                         W.h2, W.w);
-      hipLaunchKernelGGL(axpy_k, dim3(1), dim3(64 * div_up(j + 1, 64)), 0, s,
-                         (size_t)(j + 1), 1.0, W.h2, W.h);
-      GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm, 2));
-      hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.w,
-                         W.nrm, W.V + (size_t)(j + 1) * N);
-      hipLaunchKernelGGL(givens_step, dim3(1), dim3(64), 0, s, j, m, W.h,
-                         W.nrm, W.state, W.scal);
-      GF_TRY(check_launch(ctx, "arnoldi step"));
-      GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.scal, 2 * sizeof(double),
-                                 hipMemcpyDeviceToHost, s));
-      GF_HIP(ctx, hipStreamSynchronize(s));
-      j++;
-      const double est = W.host_scal[0], hn = W.host_scal[1];
-      if (!std::isfinite(est)) break;
-      if (est <= tol_abs || hn == 0.0) break;
+    // Arnoldi steps are issued one ahead of the convergence test: the host
+    // reads the residual estimate of step j while the device runs step j+1,
+    // so there is no host round trip between steps.  A step issued past the
+    // stopping point only writes column j of H, V_{j+1} and g[j..j+1], none
+    // of which enter the update with j columns.
+    int issued = 0, j = 0;
+    const int limit = std::min(m, max_iters - it);
+    bool stop = false;
+    while (!stop) {
+      if (issued < limit) {
+        const int js = issued;
+        double* vj = W.V + (size_t)js * N;
+        GF_TRY(op.apply_prec_a(vj, W.w, prec));  // w = A M^-1 v_j
+        // CGS2
+        GF_TRY(W.dots(W.V, N, js + 1, W.w, W.h, 0));
+        hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, js + 1,
+                           W.V, N, W.h, W.w);
+        GF_TRY(W.dots(W.V, N, js + 1, W.w, W.h2, 0));
+        hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, js + 1,
+                           W.V, N, W.h2, W.w);
+        hipLaunchKernelGGL(axpy_k, dim3(1), dim3(64 * div_up(js + 1, 64)), 0,
+                           s, (size_t)(js + 1), 1.0, W.h2, W.h);
+        GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm, 2));
+        hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.w,
+                           W.nrm, W.V + (size_t)(js + 1) * N);
+        hipLaunchKernelGGL(givens_step, dim3(1), dim3(64), 0, s, js, m, W.h,
+                           W.nrm, W.state, W.scal);
+        GF_TRY(check_launch(ctx, "arnoldi step"));
+        GF_HIP(ctx, hipMemcpyAsync(W.host_scal + 2 * (js % 2), W.scal,
+                                   2 * sizeof(double), hipMemcpyDeviceToHost,
+                                   s));
+        GF_HIP(ctx, hipEventRecord(W.ev[js % 2], s));
+        issued++;
+      }
+      // test step j once the next one is queued (or nothing more is issued)
+      if (issued - j >= 2 || issued == limit) {
+        GF_HIP(ctx, hipEventSynchronize(W.ev[j % 2]));
+        const double est = W.host_scal[2 * (j % 2)];
+        const double hn = W.host_scal[2 * (j % 2) + 1];
+        j++;
+        if (!std::isfinite(est) || est <= tol_abs || hn == 0.0 || j == limit)
+          stop = true;
+      }
     }
+    it += j;
+    GF_HIP(ctx, hipStreamSynchronize(s));
     // x += M^-1 V y
     hipLaunchKernelGGL(solve_upper, dim3(1), dim3(64), 0, s, j, m, W.state,
                        W.y);
