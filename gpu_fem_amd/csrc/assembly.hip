@@ -400,11 +400,11 @@ int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
       !elem_ind_map || !kvals)
     return gf::set_error(ctx, GF_ERR_INVALID, "assemble_jacobian: bad args");
   if (nelems == 0) return GF_OK;
-  static const bool per_element = [] {
+  static const bool batched = [] {
     const char* v = getenv("GPUFEM_ASSEMBLY");
-    return v && std::string(v) == "element";
+    return v && std::string(v) == "batched";
   }();
-  if (per_element) {
+  if (!batched) {
     hipLaunchKernelGGL(gf::jacobian_atomic_kernel, dim3(nelems), dim3(64), 0,
                        ctx->stream, nelems, conn, xpts, vars, data,
                        elem_ind_map, perm, kvals, res);

@@ -39,8 +39,9 @@ struct gf_ilu_s {
   int* pivot = nullptr;                    // first zero-pivot row (device)
   int max_row_len = 0;
   // sync-free sweep state
-  int *lchunk = nullptr, *uchunk = nullptr; // chunk offsets into l/urows
+  int *lchunk = nullptr, *uchunk = nullptr; // chunk offsets (see below)
   int nlchunk = 0, nuchunk = 0;
+  bool lchains = false, uchains = false;  // chunks are chains in row order
   unsigned long long* counter = nullptr;   // chunk dispenser (monotonic)
   int* err = nullptr;                      // set when a bounded wait expires
   unsigned long long base = 0;             // counter value at sweep start
@@ -200,6 +201,7 @@ __global__ void __launch_bounds__(256)
 constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 constexpr int kSweepIter = 4;             // block batches held in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
+constexpr int kCache = 4;                 // recent rows kept in registers
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
 // Sentinel values: signalling NaNs.  Arithmetic never produces a signalling
@@ -241,23 +243,24 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // relaxed agent-scope atomic loads until the value is no longer the sentinel.
 // Each entry validates itself, so no fences or flags are needed.
 //
-// Rows are taken in level order, in chunks of rows of one level (chunk
-// boundaries from the host), from a monotonic 64-bit counter.  Every
-// dependency of a row lies in an earlier level, hence in a chunk handed out
-// earlier to a running wavefront, so the sweep cannot deadlock whatever the
-// number of resident workgroups.
+// Rows are taken from a monotonic 64-bit counter in chunks, either chunks of
+// rows of one level in level order (order = the level-sorted rows), or
+// chains of consecutive rows in sweep order (order = null).  Either way every
+// dependency of a row lies in an earlier chunk, handed out earlier to a
+// running wavefront, or earlier in the same chunk, so the sweep cannot
+// deadlock whatever the number of resident workgroups.
 //
 // Lanes: slot t = lane / 6 (0..9) takes blocks k0+t, k0+t+10, ...; r =
 // lane % 6 is the scalar row; the 6x6 diagonal block is solved by lanes 0-5
-// with shuffles.  The last row solved by the wavefront is kept in registers
-// (rows of a chunk often depend on the row before).
+// with shuffles.  The last kCache rows solved by the wavefront are kept in
+// registers (the rows of a chain depend on the rows just before them).
 //
 // fill_next (lower sweep only, may be null): the entries of the row in the
 // upper sweep's output are set to the sentinel once the row's rhs is read,
 // which allows that output to alias the rhs.
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(kSweepThreads)
-    trsv_syncfree_kernel(const int* __restrict__ rowp,
+    trsv_syncfree_kernel(int n, const int* __restrict__ rowp,
                          const int* __restrict__ cols,
                          const int* __restrict__ diag,
                          const int* __restrict__ order,
@@ -269,10 +272,29 @@ __global__ void __launch_bounds__(kSweepThreads)
   const int lane = threadIdx.x % 64;
   const int t = lane / 6, r = lane % 6;
   const bool act = lane < 6 * kSlots;
-  int prev = -1;   // last row solved by this wavefront
-  T xprev[6];      // its solution, in every lane
+  // the last kCache rows solved by this wavefront and their solutions (in
+  // every lane); a chain's row usually depends on the rows just before it
+  int cid[kCache];
+  T cx[kCache][6];
 #pragma unroll
-  for (int c = 0; c < 6; c++) xprev[c] = T(0);
+  for (int d = 0; d < kCache; d++) {
+    cid[d] = -1;
+#pragma unroll
+    for (int c = 0; c < 6; c++) cx[d][c] = T(0);
+  }
+  auto cached = [&](int j) {
+    bool hit = false;
+#pragma unroll
+    for (int d = 0; d < kCache; d++) hit |= cid[d] == j;
+    return hit;
+  };
+  auto cache_value = [&](int j, int c) {
+    T v = T(0);
+#pragma unroll
+    for (int d = 0; d < kCache; d++)
+      if (cid[d] == j) v = cx[d][c];
+    return v;
+  };
   for (;;) {
     unsigned long long tk = 0;
     if (lane == 0) tk = atomicAdd(counter, 1ull);
@@ -281,7 +303,7 @@ __global__ void __launch_bounds__(kSweepThreads)
     if (kc >= nchunks) break;
     const int p0 = chunk_ptr[kc], p1 = chunk_ptr[kc + 1];
     for (int p = p0; p < p1; p++) {
-      const int i = order[p];
+      const int i = order ? order[p] : (kLower ? p : n - 1 - p);
       const int di = diag[i];
       const int k0 = kLower ? rowp[i] : di + 1;
       const int k1 = kLower ? di : rowp[i + 1];
@@ -314,8 +336,8 @@ __global__ void __launch_bounds__(kSweepThreads)
           const int j = jj[it];
 #pragma unroll
           for (int c = 0; c < 6; c++)
-            xb[it][c] = (j >= 0 && j != prev) ? load_bits(x + 6 * (size_t)j + c)
-                                              : B(0);
+            xb[it][c] = (j >= 0 && !cached(j)) ? load_bits(x + 6 * (size_t)j + c)
+                                                : B(0);
         }
         unsigned spins = 0;
         for (;;) {
@@ -348,8 +370,8 @@ __global__ void __launch_bounds__(kSweepThreads)
         for (int it = 0; it < kSweepIter; it++)
 #pragma unroll
           for (int c = 0; c < 6; c++)
-            xv[it][c] = jj[it] == prev && jj[it] >= 0
-                            ? xprev[c]
+            xv[it][c] = jj[it] >= 0 && cached(jj[it])
+                            ? cache_value(jj[it], c)
                             : __builtin_bit_cast(T, xb[it][c]);
 #pragma unroll
         for (int it = 0; it < kSweepIter; it++)
@@ -377,8 +399,14 @@ __global__ void __launch_bounds__(kSweepThreads)
       }
       if (lane < 6) store_value(x + 6 * (size_t)i + lane, s);
 #pragma unroll
-      for (int c = 0; c < 6; c++) xprev[c] = __shfl(s, c, 64);
-      prev = i;
+      for (int d = kCache - 1; d > 0; d--) {
+        cid[d] = cid[d - 1];
+#pragma unroll
+        for (int c = 0; c < 6; c++) cx[d][c] = cx[d - 1][c];
+      }
+      cid[0] = i;
+#pragma unroll
+      for (int c = 0; c < 6; c++) cx[0][c] = __shfl(s, c, 64);
     }
   }
 }
@@ -429,9 +457,32 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
         out.push_back(std::min(a + csz, lp[l + 1]));
     }
   };
-  std::vector<int> lch, uch;
-  chunks(h->lptr, lch);
-  chunks(h->uptr, uch);
+  // chains: maximal runs of consecutive rows (in sweep order) where each row
+  // depends on the one before it
+  auto chains = [&](bool lower, std::vector<int>& out) {
+    out.assign(1, 0);
+    for (int t = 1; t < n; t++) {
+      const int i = lower ? t : n - 1 - t;
+      bool dep = false;
+      if (lower) {
+        for (int k = rowp[i]; k < diag[i]; k++) dep |= cols[k] == i - 1;
+      } else {
+        for (int k = diag[i] + 1; k < rowp[i + 1]; k++) dep |= cols[k] == i + 1;
+      }
+      if (!dep) out.push_back(t);
+    }
+    out.push_back(n);
+  };
+  // Long chains (banded orders: natural, RCM) are handed out whole, in row
+  // order, so each wavefront forwards a chain's rows in registers; short
+  // chains (AMD) are handed out by level, in chunks of rows of one level.
+  std::vector<int> lch, uch, lcn, ucn;
+  chains(true, lcn);
+  chains(false, ucn);
+  h->lchains = n >= 4 * ((int)lcn.size() - 1);
+  h->uchains = n >= 4 * ((int)ucn.size() - 1);
+  if (h->lchains) lch = lcn; else chunks(h->lptr, lch);
+  if (h->uchains) uch = ucn; else chunks(h->uptr, uch);
   h->nlchunk = (int)lch.size() - 1;
   h->nuchunk = (int)uch.size() - 1;
   auto up = [&](int** dst, const int* src, size_t cnt) -> int {
@@ -494,9 +545,11 @@ int gf_ilu_destroy(gf_ilu h) {
 int gf_ilu_info(gf_ilu h, int* nnzb, int* nl, int* nu) {
   if (!h) return GF_ERR_INVALID;
   if (getenv("GPUFEM_ILU_VERBOSE"))
-    fprintf(stderr, "ilu: n=%d nnzb=%d levels L=%d U=%d chunks L=%d U=%d\n",
-            h->n, h->nnzb, (int)h->lptr.size() - 1, (int)h->uptr.size() - 1,
-            h->nlchunk, h->nuchunk);
+    fprintf(stderr,
+            "ilu: n=%d nnzb=%d levels L=%d U=%d %s L=%d %s U=%d\n", h->n,
+            h->nnzb, (int)h->lptr.size() - 1, (int)h->uptr.size() - 1,
+            h->lchains ? "chains" : "chunks", h->nlchunk,
+            h->uchains ? "chains" : "chunks", h->nuchunk);
   if (nnzb) *nnzb = h->nnzb;
   if (nl) *nl = (int)h->lptr.size() - 1;
   if (nu) *nu = (int)h->uptr.size() - 1;
@@ -560,14 +613,16 @@ int ilu_apply(gf_ilu h, const T* lu, const T* b, T* tmp, T* x) {
     hipLaunchKernelGGL(gf::fill_sentinel_kernel<T>, dim3(gf::div_up(N, 256)),
                        dim3(256), 0, ctx->stream, N, tmp);
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->rowp,
-                       h->cols, h->diag, h->lrows, h->lchunk, h->nlchunk, lu,
-                       b, tmp, x, h->counter, h->base, h->err);
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
+                       h->cols, h->diag, h->lchains ? nullptr : h->lrows,
+                       h->lchunk, h->nlchunk, lu, b, tmp, x, h->counter,
+                       h->base, h->err);
     h->base += (unsigned long long)h->nlchunk + waves;
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->rowp,
-                       h->cols, h->diag, h->urows, h->uchunk, h->nuchunk, lu,
-                       tmp, x, (T*)nullptr, h->counter, h->base, h->err);
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
+                       h->cols, h->diag, h->uchains ? nullptr : h->urows,
+                       h->uchunk, h->nuchunk, lu, tmp, x, (T*)nullptr,
+                       h->counter, h->base, h->err);
     h->base += (unsigned long long)h->nuchunk + waves;
     return gf::check_launch(ctx, "trsv_syncfree_kernel");
   }
