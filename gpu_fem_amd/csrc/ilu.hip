@@ -40,6 +40,8 @@ struct gf_ilu_s {
   int max_row_len = 0;
   // sync-free sweep state
   int *lchunk = nullptr, *uchunk = nullptr; // chunk offsets (see below)
+  int4 *lmeta = nullptr, *umeta = nullptr;  // {row, k0, k1, diag} per
+                                            // position in hand-out order
   int nlchunk = 0, nuchunk = 0;
   bool lchains = false, uchains = false;  // chunks are chains in row order
   unsigned long long* counter = nullptr;   // chunk dispenser (monotonic)
@@ -260,10 +262,8 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // which allows that output to alias the rhs.
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(kSweepThreads)
-    trsv_syncfree_kernel(int n, const int* __restrict__ rowp,
+    trsv_syncfree_kernel(const int4* __restrict__ meta,
                          const int* __restrict__ cols,
-                         const int* __restrict__ diag,
-                         const int* __restrict__ order,
                          const int* __restrict__ chunk_ptr, int nchunks,
                          const T* __restrict__ lu, const T* b, T* x,
                          T* fill_next, unsigned long long* counter,
@@ -295,18 +295,20 @@ __global__ void __launch_bounds__(kSweepThreads)
       if (cid[d] == j) v = cx[d][c];
     return v;
   };
-  for (;;) {
-    unsigned long long tk = 0;
-    if (lane == 0) tk = atomicAdd(counter, 1ull);
-    tk = __shfl(tk, 0, 64);
-    const long long kc = (long long)(tk - base);
-    if (kc >= nchunks) break;
+  unsigned long long tk = 0;
+  if (lane == 0) tk = atomicAdd(counter, 1ull);
+  tk = __shfl(tk, 0, 64);
+  long long kc = (long long)(tk - base);
+  while (kc < nchunks) {
+    // take the next chunk now: the counter round trip overlaps this chunk
+    unsigned long long tn = 0;
+    if (lane == 0) tn = atomicAdd(counter, 1ull);
     const int p0 = chunk_ptr[kc], p1 = chunk_ptr[kc + 1];
+    int4 mnext = meta[p0];
     for (int p = p0; p < p1; p++) {
-      const int i = order ? order[p] : (kLower ? p : n - 1 - p);
-      const int di = diag[i];
-      const int k0 = kLower ? rowp[i] : di + 1;
-      const int k1 = kLower ? di : rowp[i + 1];
+      const int4 m = mnext;
+      if (p + 1 < p1) mnext = meta[p + 1];
+      const int i = m.x, k0 = m.y, k1 = m.z, di = m.w;
       T s = (lane < 6) ? b[6 * (size_t)i + lane] : T(0);
       if (kLower && fill_next && lane < 6)
         reinterpret_cast<B*>(fill_next)[6 * (size_t)i + lane] =
@@ -408,6 +410,8 @@ __global__ void __launch_bounds__(kSweepThreads)
 #pragma unroll
       for (int c = 0; c < 6; c++) cx[0][c] = __shfl(s, c, 64);
     }
+    tn = __shfl(tn, 0, 64);
+    kc = (long long)(tn - base);
   }
 }
 }  // namespace
@@ -473,18 +477,32 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
     }
     out.push_back(n);
   };
-  // Long chains (banded orders: natural, RCM) are handed out whole, in row
-  // order, so each wavefront forwards a chain's rows in registers; short
-  // chains (AMD) are handed out by level, in chunks of rows of one level.
+  // Rows are handed out by level, in chunks of rows of one level.  Handing
+  // out whole chains in row order instead (GPUFEM_TRSV=chains; each
+  // wavefront forwards a chain's rows in registers) serialises the rows of a
+  // chain behind each other's memory latency and measured 1.7x slower on the
+  // natural-order plates.
   std::vector<int> lch, uch, lcn, ucn;
-  chains(true, lcn);
-  chains(false, ucn);
-  h->lchains = n >= 4 * ((int)lcn.size() - 1);
-  h->uchains = n >= 4 * ((int)ucn.size() - 1);
+  const char* tenv = getenv("GPUFEM_TRSV");
+  const bool use_chains = tenv && std::string(tenv) == "chains";
+  if (use_chains) {
+    chains(true, lcn);
+    chains(false, ucn);
+  }
+  h->lchains = use_chains;
+  h->uchains = use_chains;
   if (h->lchains) lch = lcn; else chunks(h->lptr, lch);
   if (h->uchains) uch = ucn; else chunks(h->uptr, uch);
   h->nlchunk = (int)lch.size() - 1;
   h->nuchunk = (int)uch.size() - 1;
+  // row descriptors in hand-out order: one 16-byte load per row
+  std::vector<int4> lm(n), um(n);
+  for (int p = 0; p < n; p++) {
+    const int li = use_chains ? p : lrows[p];
+    lm[p] = make_int4(li, rowp[li], diag[li], diag[li]);
+    const int ui = use_chains ? n - 1 - p : urows[p];
+    um[p] = make_int4(ui, diag[ui] + 1, rowp[ui + 1], diag[ui]);
+  }
   auto up = [&](int** dst, const int* src, size_t cnt) -> int {
     GF_HIP(ctx, hipMalloc(dst, sizeof(int) * (cnt ? cnt : 1)));
     GF_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(int) * cnt,
@@ -497,6 +515,8 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
       (st = up(&h->lrows, lrows.data(), n)) ||
       (st = up(&h->urows, urows.data(), n)) ||
       (st = up(&h->lchunk, lch.data(), lch.size())) ||
+      (st = up((int**)&h->lmeta, (const int*)lm.data(), 4 * (size_t)n)) ||
+      (st = up((int**)&h->umeta, (const int*)um.data(), 4 * (size_t)n)) ||
       (st = up(&h->uchunk, uch.data(), uch.size())) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->pivot, sizeof(int)), "malloc")) ||
       (st = gf::check_hip(ctx, hipMalloc(&h->counter, sizeof(unsigned long long)),
@@ -535,6 +555,8 @@ int gf_ilu_destroy(gf_ilu h) {
   hipFree(h->urows);
   hipFree(h->pivot);
   hipFree(h->lchunk);
+  hipFree(h->lmeta);
+  hipFree(h->umeta);
   hipFree(h->uchunk);
   hipFree(h->counter);
   hipFree(h->err);
@@ -613,15 +635,13 @@ int ilu_apply(gf_ilu h, const T* lu, const T* b, T* tmp, T* x) {
     hipLaunchKernelGGL(gf::fill_sentinel_kernel<T>, dim3(gf::div_up(N, 256)),
                        dim3(256), 0, ctx->stream, N, tmp);
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
-                       h->cols, h->diag, h->lchains ? nullptr : h->lrows,
-                       h->lchunk, h->nlchunk, lu, b, tmp, x, h->counter,
-                       h->base, h->err);
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->lmeta,
+                       h->cols, h->lchunk, h->nlchunk, lu, b, tmp, x,
+                       h->counter, h->base, h->err);
     h->base += (unsigned long long)h->nlchunk + waves;
     hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->n, h->rowp,
-                       h->cols, h->diag, h->uchains ? nullptr : h->urows,
-                       h->uchunk, h->nuchunk, lu, tmp, x, (T*)nullptr,
+                       dim3(gf::kSweepThreads), 0, ctx->stream, h->umeta,
+                       h->cols, h->uchunk, h->nuchunk, lu, tmp, x, (T*)nullptr,
                        h->counter, h->base, h->err);
     h->base += (unsigned long long)h->nuchunk + waves;
     return gf::check_launch(ctx, "trsv_syncfree_kernel");

@@ -15,13 +15,16 @@ struct KrylovWork {
          *tmp = nullptr;
   double *h = nullptr, *h2 = nullptr, *nrm = nullptr, *y = nullptr;
   double *state = nullptr, *partial = nullptr, *scal = nullptr;
+  int* gate = nullptr;  // second Gram-Schmidt pass needed (device flag)
   float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
   double* host_scal = nullptr;  // pinned: two (estimate, h_{j+1,j}) slots
   hipEvent_t ev[2] = {nullptr, nullptr};
   int init(gf_context c, size_t N, int m, bool mixed);
   void release();
+  // out[0..nvec) = V_i . x (mode 0 store, 1 add, 2 sqrt); skipped when
+  // *gate == 0 (a stored result then reads zero)
   int dots(const double* V, size_t ldv, int nvec, const double* x, double* out,
-           int mode);
+           int mode, const int* gate = nullptr);
 };
 
 // The linear operator A (BSR, FP64 and optional FP32 copy) and the ILU

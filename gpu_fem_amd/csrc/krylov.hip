@@ -27,8 +27,10 @@ constexpr int kDotChunk = 8;
 __global__ void __launch_bounds__(kDotThreads)
     multi_dot_partial(size_t N, int nvec, const double* __restrict__ V,
                       size_t ldv, const double* __restrict__ w,
-                      double* __restrict__ partial) {
+                      double* __restrict__ partial,
+                      const int* __restrict__ gate) {
   __shared__ double red[kDotThreads / 64][kDotChunk];
+  if (gate && *gate == 0) return;
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   for (int i0 = 0; i0 < nvec; i0 += kDotChunk) {
     double acc[kDotChunk];
@@ -61,9 +63,14 @@ __global__ void __launch_bounds__(kDotThreads)
 // One workgroup per i; fixed summation tree (deterministic).
 __global__ void __launch_bounds__(kDotThreads)
     multi_dot_final(int nblocks, int nvec, const double* __restrict__ partial,
-                    double* __restrict__ out, int mode) {
+                    double* __restrict__ out, int mode,
+                    const int* __restrict__ gate) {
   __shared__ double red[kDotThreads / 64];
   const int i = blockIdx.x;
+  if (gate && *gate == 0) {  // pass skipped: a stored result reads zero
+    if (threadIdx.x == 0 && mode == 0) out[i] = 0.0;
+    return;
+  }
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   double s = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += kDotThreads)
@@ -83,15 +90,24 @@ __global__ void __launch_bounds__(kDotThreads)
   }
 }
 
-// w -= sum_i h[i] V_i
+// w -= sum_i h[i] V_i (skipped when *gate == 0)
 __global__ void update_w(size_t N, int nvec, const double* __restrict__ V,
                          size_t ldv, const double* __restrict__ h,
-                         double* __restrict__ w) {
+                         double* __restrict__ w, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
   double s = w[t];
   for (int i = 0; i < nvec; i++) s -= h[i] * V[(size_t)i * ldv + t];
   w[t] = s;
+}
+
+// second Gram-Schmidt pass only if the first one cancelled more than
+// 1 - 1/sqrt(2) of w (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30,
+// 1976): gate = ||w_after|| < ||w_before|| / sqrt(2)
+__global__ void reorth_gate(const double* __restrict__ nrm, int* gate) {
+  if (threadIdx.x == 0)
+    *gate = nrm[0] < 0.70710678118654752 * nrm[1] ? 1 : 0;
 }
 
 // u = sum_i y[i] V_i
@@ -212,6 +228,7 @@ int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
   GF_TRY(dm((void**)&state, sizeof(double) * ((m + 1) * m + 3 * m + 2)));
   GF_TRY(dm((void**)&partial, sizeof(double) * kDotBlocks * (m + 2)));
   GF_TRY(dm((void**)&scal, sizeof(double) * 4));
+  GF_TRY(dm((void**)&gate, sizeof(int)));
   if (mixed) {
     GF_TRY(dm((void**)&a32, sizeof(float) * N));
     GF_TRY(dm((void**)&b32, sizeof(float) * N));
@@ -239,6 +256,7 @@ void KrylovWork::release() {
   hipFree(state);
   hipFree(partial);
   hipFree(scal);
+  hipFree(gate);
   hipFree(a32);
   hipFree(b32);
   hipFree(c32);
@@ -250,11 +268,12 @@ void KrylovWork::release() {
 
 // out[0..nvec) = V_i . w  (mode: 0 store, 1 add, 2 sqrt)
 int KrylovWork::dots(const double* Vb, size_t ldv, int nvec, const double* x,
-                     double* out, int mode) {
+                     double* out, int mode, const int* gate) {
   hipLaunchKernelGGL(multi_dot_partial, dim3(kDotBlocks), dim3(kDotThreads), 0,
-                     ctx->stream, N, nvec, Vb, ldv, x, partial);
+                     ctx->stream, N, nvec, Vb, ldv, x, partial, gate);
   hipLaunchKernelGGL(multi_dot_final, dim3(nvec), dim3(kDotThreads), 0,
-                     ctx->stream, (int)kDotBlocks, nvec, partial, out, mode);
+                     ctx->stream, (int)kDotBlocks, nvec, partial, out, mode,
+                     gate);
   return check_launch(ctx, "multi_dot");
 }
 
@@ -316,16 +335,20 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
         const int js = issued;
         double* vj = W.V + (size_t)js * N;
         GF_TRY(op.apply_prec_a(vj, W.w, prec));  // w = A M^-1 v_j
-        // CGS2
+        // classical Gram-Schmidt, second pass only when needed (DGKS)
+        GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm + 1, 2));  // ||w|| before
         GF_TRY(W.dots(W.V, N, js + 1, W.w, W.h, 0));
         hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, js + 1,
-                           W.V, N, W.h, W.w);
-        GF_TRY(W.dots(W.V, N, js + 1, W.w, W.h2, 0));
+                           W.V, N, W.h, W.w, (const int*)nullptr);
+        GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm, 2));
+        hipLaunchKernelGGL(reorth_gate, dim3(1), dim3(64), 0, s, W.nrm,
+                           W.gate);
+        GF_TRY(W.dots(W.V, N, js + 1, W.w, W.h2, 0, W.gate));
         hipLaunchKernelGGL(update_w, grid1(N), dim3(256), 0, s, N, js + 1,
-                           W.V, N, W.h2, W.w);
+                           W.V, N, W.h2, W.w, (const int*)W.gate);
         hipLaunchKernelGGL(axpy_k, dim3(1), dim3(64 * div_up(js + 1, 64)), 0,
                            s, (size_t)(js + 1), 1.0, W.h2, W.h);
-        GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm, 2));
+        GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm, 2, W.gate));
         hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.w,
                            W.nrm, W.V + (size_t)(js + 1) * N);
         hipLaunchKernelGGL(givens_step, dim3(1), dim3(64), 0, s, js, m, W.h,
