@@ -72,6 +72,11 @@ _SIGS = {
     "gf_assemble_jacobian_mitc4_f64": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P,
                                        _P],
     "gf_assemble_residual_mitc4_f64": [_P, _I, _I, _P, _P, _P, _P, _P, _P],
+    "gf_asm_plan_create": [_P, _I, _I, _P, _P, _I, ctypes.POINTER(_P)],
+    "gf_asm_plan_destroy": [_P],
+    "gf_asm_plan_colors": [_P],
+    "gf_assemble_jacobian_colored_f64": [_P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                         _P],
     "gf_energy_mitc4_f64": [_P, _I, _P, _P, _P, _P, _P],
     "gf_bc_apply_mat": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "gf_bc_apply_vec": [_P, _I, _P, _P, _P],

@@ -14,8 +14,21 @@
 #include <cstdlib>
 #include <string>
 
+#include <vector>
+
 #include "common.h"
 #include "mitc4.h"
+#include "symbolic.h"
+
+struct gf_asm_plan_s {
+  gf_context ctx = nullptr;
+  int nelems = 0, ncolors = 0;
+  std::vector<int> color_ptr;
+  int* elems = nullptr;  // elements in colour-major order (device)
+  int* mask = nullptr;   // first-touch masks in the same order (device)
+  bool zero_k = false, zero_res = false;  // untouched K blocks / rows exist
+  int nnzb = 0, nnodes = 0;
+};
 
 namespace gf {
 namespace {
@@ -107,18 +120,28 @@ __device__ __forceinline__ void element_tile(const ElemLDS& s, int I, int J,
     }
 }
 
+// kColored = false: every contribution is an FP64 atomic add (K and res
+// zeroed by the caller).  kColored = true: the launch covers one colour of
+// an element colouring (no two elements of a colour share a node), so no two
+// lanes of the launch touch the same K entry or residual entry; each
+// contribution is a plain store when the element is the first to touch the
+// block (first-touch mask, bits 4a+b for K blocks, 16+a for residual rows)
+// and a plain read-add-write otherwise.  No atomics and no zeroing pass.
+template <bool kColored>
 __global__ void __launch_bounds__(64)
-    jacobian_atomic_kernel(int nelems, const int* __restrict__ conn,
-                           const double* __restrict__ xpts,
-                           const double* __restrict__ vars,
-                           const double* __restrict__ data,
-                           const int* __restrict__ elem_ind_map,
-                           const int* __restrict__ perm, double* kvals,
-                           double* res) {
+    jacobian_kernel(int nelems, const int* __restrict__ elem_list,
+                    const int* __restrict__ first_mask,
+                    const int* __restrict__ conn,
+                    const double* __restrict__ xpts,
+                    const double* __restrict__ vars,
+                    const double* __restrict__ data,
+                    const int* __restrict__ elem_ind_map,
+                    const int* __restrict__ perm, double* kvals, double* res) {
   __shared__ ElemLDS s;
-  const int e = blockIdx.x;
+  if ((int)blockIdx.x >= nelems) return;
+  const int e = kColored ? elem_list[blockIdx.x] : (int)blockIdx.x;
+  const int mask = kColored ? first_mask[blockIdx.x] : 0;
   const int lane = threadIdx.x;
-  if (e >= nelems) return;
   load_element(s, e, lane, conn, xpts, vars, data);
   __syncthreads();
   element_geometry(s, lane);
@@ -130,11 +153,17 @@ __global__ void __launch_bounds__(64)
   const int a = I / 2, b = J / 2, r0 = 3 * (I % 2), c0 = 3 * (J % 2);
   const size_t blk = (size_t)elem_ind_map[16 * (size_t)e + 4 * a + b];
   double* dst = kvals + 36 * blk;
+  const bool first = (mask >> (4 * a + b)) & 1;
 #pragma unroll
   for (int r = 0; r < 3; r++)
 #pragma unroll
-    for (int c = 0; c < 3; c++)
-      unsafeAtomicAdd(&dst[6 * (r0 + r) + c0 + c], acc[r][c]);
+    for (int c = 0; c < 3; c++) {
+      double* d = &dst[6 * (r0 + r) + c0 + c];
+      if (!kColored)
+        unsafeAtomicAdd(d, acc[r][c]);
+      else
+        *d = first ? acc[r][c] : *d + acc[r][c];
+    }
   if (res) {
     double p0 = 0, p1 = 0, p2 = 0;
 #pragma unroll
@@ -154,7 +183,11 @@ __global__ void __launch_bounds__(64)
       for (int j = 0; j < 8; j++) r += s.rpart[j][lane];
       const int node = conn[4 * (size_t)e + lane / 6];
       const int row = perm ? perm[node] : node;
-      unsafeAtomicAdd(&res[6 * (size_t)row + lane % 6], r);
+      double* d = &res[6 * (size_t)row + lane % 6];
+      if (!kColored)
+        unsafeAtomicAdd(d, r);
+      else
+        *d = ((mask >> (16 + lane / 6)) & 1) ? r : *d + r;
     }
   }
 }
@@ -405,16 +438,108 @@ int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
     return v && std::string(v) == "batched";
   }();
   if (!batched) {
-    hipLaunchKernelGGL(gf::jacobian_atomic_kernel, dim3(nelems), dim3(64), 0,
-                       ctx->stream, nelems, conn, xpts, vars, data,
+    hipLaunchKernelGGL(gf::jacobian_kernel<false>, dim3(nelems), dim3(64), 0,
+                       ctx->stream, nelems, (const int*)nullptr,
+                       (const int*)nullptr, conn, xpts, vars, data,
                        elem_ind_map, perm, kvals, res);
-    return gf::check_launch(ctx, "jacobian_atomic_kernel");
+    return gf::check_launch(ctx, "jacobian_kernel");
   }
   hipLaunchKernelGGL(gf::jacobian_batched_kernel,
                      dim3(gf::div_up(nelems, gf::kElemsPerWave)), dim3(64), 0,
                      ctx->stream, nelems, conn, xpts, vars, data, elem_ind_map,
                      perm, kvals, res);
   return gf::check_launch(ctx, "jacobian_batched_kernel");
+}
+
+int gf_asm_plan_create(gf_context ctx, int nelems, int nnodes,
+                       const int* conn, const int* elem_ind_map, int nnzb,
+                       gf_asm_plan* out) {
+  if (!ctx || nelems <= 0 || nnodes <= 0 || !conn || !elem_ind_map ||
+      nnzb <= 0 || !out)
+    return gf::set_error(ctx, GF_ERR_INVALID, "asm_plan_create: bad args");
+  auto* pl = new gf_asm_plan_s;
+  pl->ctx = ctx;
+  pl->nelems = nelems;
+  pl->nnzb = nnzb;
+  pl->nnodes = nnodes;
+  std::vector<int> elems;
+  gf::element_coloring(nelems, nnodes, 4, conn, pl->color_ptr, elems);
+  pl->ncolors = (int)pl->color_ptr.size() - 1;
+  // first-touch masks in processing order (colour-major)
+  std::vector<int> mask(nelems, 0);
+  std::vector<char> kt(nnzb, 0), rt(nnodes, 0);
+  for (int p = 0; p < nelems; p++) {
+    const int e = elems[p];
+    int m = 0;
+    for (int ab = 0; ab < 16; ab++) {
+      const int blk = elem_ind_map[16 * (size_t)e + ab];
+      if (blk < 0 || blk >= nnzb) {
+        delete pl;
+        return gf::set_error(ctx, GF_ERR_INVALID,
+                             "asm_plan_create: element %d maps outside K", e);
+      }
+      if (!kt[blk]) {
+        kt[blk] = 1;
+        m |= 1 << ab;
+      }
+    }
+    for (int a = 0; a < 4; a++) {
+      const int nd = conn[4 * (size_t)e + a];
+      if (!rt[nd]) {
+        rt[nd] = 1;
+        m |= 1 << (16 + a);
+      }
+    }
+    mask[p] = m;
+  }
+  for (int k = 0; k < nnzb && !pl->zero_k; k++) pl->zero_k = !kt[k];
+  for (int i = 0; i < nnodes && !pl->zero_res; i++) pl->zero_res = !rt[i];
+  if (hipMalloc(&pl->elems, sizeof(int) * nelems) != hipSuccess ||
+      hipMalloc(&pl->mask, sizeof(int) * nelems) != hipSuccess) {
+    gf_asm_plan_destroy(pl);
+    return gf::set_error(ctx, GF_ERR_NOMEM, "asm_plan_create: hipMalloc");
+  }
+  if (hipMemcpy(pl->elems, elems.data(), sizeof(int) * nelems,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(pl->mask, mask.data(), sizeof(int) * nelems,
+                hipMemcpyHostToDevice) != hipSuccess) {
+    gf_asm_plan_destroy(pl);
+    return gf::set_error(ctx, GF_ERR_DEVICE, "asm_plan_create: upload");
+  }
+  *out = pl;
+  return GF_OK;
+}
+
+int gf_asm_plan_destroy(gf_asm_plan pl) {
+  if (!pl) return GF_OK;
+  hipFree(pl->elems);
+  hipFree(pl->mask);
+  delete pl;
+  return GF_OK;
+}
+
+int gf_asm_plan_colors(gf_asm_plan pl) { return pl ? pl->ncolors : -1; }
+
+int gf_assemble_jacobian_colored_f64(gf_context ctx, gf_asm_plan pl,
+                                     const int* conn, const double* xpts,
+                                     const double* vars, const double* data,
+                                     const int* elem_ind_map, const int* perm,
+                                     double* kvals, double* res) {
+  if (!ctx || !pl || !conn || !xpts || !data || !elem_ind_map || !kvals)
+    return gf::set_error(ctx, GF_ERR_INVALID, "assemble_colored: bad args");
+  if (pl->zero_k)
+    GF_HIP(ctx, hipMemsetAsync(kvals, 0, sizeof(double) * 36 * (size_t)pl->nnzb,
+                               ctx->stream));
+  if (res && pl->zero_res)
+    GF_HIP(ctx, hipMemsetAsync(res, 0, sizeof(double) * 6 * (size_t)pl->nnodes,
+                               ctx->stream));
+  for (int c = 0; c < pl->ncolors; c++) {
+    const int c0 = pl->color_ptr[c], cnt = pl->color_ptr[c + 1] - c0;
+    hipLaunchKernelGGL(gf::jacobian_kernel<true>, dim3(cnt), dim3(64), 0,
+                       ctx->stream, cnt, pl->elems + c0, pl->mask + c0, conn,
+                       xpts, vars, data, elem_ind_map, perm, kvals, res);
+  }
+  return gf::check_launch(ctx, "jacobian_kernel<colored>");
 }
 
 int gf_assemble_residual_mitc4_f64(gf_context ctx, int nelems, int nnodes,

@@ -87,6 +87,7 @@ struct gf_problem_s {
   double *work64 = nullptr, *work64b = nullptr;
   float *t32a = nullptr, *t32b = nullptr, *t32c = nullptr;
   gf_ilu ilu = nullptr;
+  gf_asm_plan plan = nullptr;  // element colouring for the atomic-free scatter
   gf::KrylovWork kw;
   hipEvent_t ev[2] = {nullptr, nullptr};
   double t_assemble = 0, t_factor = 0;
@@ -155,6 +156,8 @@ int problem_setup(gf_problem p, const int* conn, const double* xpts,
   GF_TRY(upload(ctx, &p->krowp, krp.data(), krp.size()));
   GF_TRY(upload(ctx, &p->kcols, kcl.data(), kcl.size()));
   GF_TRY(upload(ctx, &p->emap, emap.data(), emap.size()));
+  GF_TRY(gf_asm_plan_create(ctx, ne, n, conn, emap.data(), p->nnzb_k,
+                            &p->plan));
   GF_TRY(upload(ctx, &p->trowp, tr.data(), tr.size()));
   GF_TRY(upload(ctx, &p->tcols, tc.data(), tc.size()));
   GF_TRY(upload(ctx, &p->tmap, tm.data(), tm.size()));
@@ -197,6 +200,7 @@ int gf_problem_destroy(gf_problem p) {
   for (void* b : bufs)
     if (b) hipFree(b);
   if (p->ilu) gf_ilu_destroy(p->ilu);
+  if (p->plan) gf_asm_plan_destroy(p->plan);
   p->kw.release();
   if (p->ev[0]) hipEventDestroy(p->ev[0]);
   if (p->ev[1]) hipEventDestroy(p->ev[1]);
@@ -239,11 +243,9 @@ int gf_problem_assemble(gf_problem p) {
   if (!p) return GF_ERR_INVALID;
   gf_context ctx = p->ctx;
   GF_HIP(ctx, hipEventRecord(p->ev[0], ctx->stream));
-  GF_HIP(ctx, hipMemsetAsync(p->kvals, 0, sizeof(double) * 36 * p->nnzb_k,
-                             ctx->stream));
-  GF_TRY(gf_assemble_jacobian_mitc4_f64(ctx, p->nelems, p->nnodes, p->conn,
-                                        p->xpts, nullptr, p->data, p->emap,
-                                        p->perm, p->kvals, nullptr));
+  GF_TRY(gf_assemble_jacobian_colored_f64(ctx, p->plan, p->conn, p->xpts,
+                                          nullptr, p->data, p->emap, p->perm,
+                                          p->kvals, nullptr));
   GF_TRY(gf_bc_apply_mat(ctx, p->nbcs, p->bcs, p->krowp, p->kcols, p->trowp,
                          p->tcols, p->tmap, p->perm, p->kvals));
   GF_TRY(gf_bsr_copy_to_pattern(ctx, p->nnodes, p->krowp, p->kcols, p->kvals,

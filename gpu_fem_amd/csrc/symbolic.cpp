@@ -270,6 +270,42 @@ void level_schedule(int n, const int* rowp, const int* cols, bool lower,
   }
 }
 
+void element_coloring(int nelems, int nnodes, int npe, const int* conn,
+                      std::vector<int>& color_ptr,
+                      std::vector<int>& color_elems) {
+  // node -> elements
+  std::vector<int> nptr(nnodes + 1, 0), nel((size_t)nelems * npe);
+  for (size_t k = 0; k < (size_t)nelems * npe; k++) nptr[conn[k] + 1]++;
+  for (int i = 0; i < nnodes; i++) nptr[i + 1] += nptr[i];
+  {
+    std::vector<int> pos(nptr.begin(), nptr.end() - 1);
+    for (int e = 0; e < nelems; e++)
+      for (int a = 0; a < npe; a++) nel[pos[conn[(size_t)npe * e + a]]++] = e;
+  }
+  std::vector<int> color(nelems, -1), mark;
+  int ncolors = 0;
+  for (int e = 0; e < nelems; e++) {
+    mark.assign(ncolors + 1, 0);
+    for (int a = 0; a < npe; a++) {
+      const int nd = conn[(size_t)npe * e + a];
+      for (int k = nptr[nd]; k < nptr[nd + 1]; k++) {
+        const int c = color[nel[k]];
+        if (c >= 0) mark[c] = 1;
+      }
+    }
+    int c = 0;
+    while (mark[c]) c++;
+    color[e] = c;
+    ncolors = std::max(ncolors, c + 1);
+  }
+  color_ptr.assign(ncolors + 1, 0);
+  for (int e = 0; e < nelems; e++) color_ptr[color[e] + 1]++;
+  for (int c = 0; c < ncolors; c++) color_ptr[c + 1] += color_ptr[c];
+  color_elems.resize(nelems);
+  std::vector<int> pos(color_ptr.begin(), color_ptr.end() - 1);
+  for (int e = 0; e < nelems; e++) color_elems[pos[color[e]]++] = e;
+}
+
 }  // namespace gf
 
 // ------------------------------------------------------------------ C ABI

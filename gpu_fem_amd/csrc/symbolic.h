@@ -34,4 +34,12 @@ int bandwidth(int n, const int* rowp, const int* cols);
 // rows grouped by dependency level for the lower (L) or upper (U) sweep
 void level_schedule(int n, const int* rowp, const int* cols, bool lower,
                     std::vector<int>& level_ptr, std::vector<int>& level_rows);
+// Greedy element colouring: no two elements of a colour share a node.
+// Elements are visited in index order and take the smallest colour unused by
+// the elements already coloured around their nodes.  Returns the colour
+// offsets (color_ptr) and the elements in colour-major, index order
+// (color_elems).
+void element_coloring(int nelems, int nnodes, int nodes_per_elem,
+                      const int* conn, std::vector<int>& color_ptr,
+                      std::vector<int>& color_elems);
 }  // namespace gf

@@ -97,6 +97,25 @@ int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
                                    const int* d_elem_ind_map,
                                    const int* d_perm, double* d_kvals,
                                    double* d_res);
+/* Conflict-free variant of the Jacobian scatter: the elements are coloured
+ * on the host (no two elements of a colour share a node) and each colour is
+ * one launch whose contributions are plain stores (first element to touch a
+ * block) or plain read-add-writes, so K and res need no zeroing and no
+ * atomics are used.  The plan holds the colouring and the first-touch masks
+ * for one connectivity + elemIndMap (host arrays; nnzb = K blocks).  Results
+ * equal gf_assemble_jacobian_mitc4_f64 on zeroed K/res up to the order of
+ * the (at most four per entry for quads) FP64 additions. */
+typedef struct gf_asm_plan_s* gf_asm_plan;
+int gf_asm_plan_create(gf_context ctx, int nelems, int nnodes, const int* conn,
+                       const int* elem_ind_map, int nnzb, gf_asm_plan* out);
+int gf_asm_plan_destroy(gf_asm_plan plan);
+int gf_asm_plan_colors(gf_asm_plan plan);
+int gf_assemble_jacobian_colored_f64(gf_context ctx, gf_asm_plan plan,
+                                     const int* d_conn, const double* d_xpts,
+                                     const double* d_vars, const double* d_data,
+                                     const int* d_elem_ind_map,
+                                     const int* d_perm, double* d_kvals,
+                                     double* d_res);
 /* residual only (assembler.h:232 add_residual) */
 int gf_assemble_residual_mitc4_f64(gf_context ctx, int nelems, int nnodes,
                                    const int* d_conn, const double* d_xpts,

@@ -78,6 +78,25 @@ def test_jacobian_and_residual(gpu_ctx, name):
     res = host(d_r)
     assert np.abs(res - res_o).max() <= 1e-12 * np.abs(res_o).max()
 
+    # coloured, atomic-free scatter: K and res start dirty (no zeroing needed)
+    import ctypes
+    plan = ctypes.c_void_p()
+    gpu_ctx.check(G.lib().gf_asm_plan_create(
+        gpu_ctx.h, m.nelems, m.nnodes, G.ptr(np.ascontiguousarray(m.conn, np.int32)),
+        G.ptr(np.ascontiguousarray(s["emap"], np.int32)), nnzb, ctypes.byref(plan)))
+    assert 2 <= G.lib().gf_asm_plan_colors(plan) <= 16
+    d_k2 = torch.full_like(d_k, 3.0)
+    d_rc = torch.full_like(d_r, -2.0)
+    gpu_ctx.check(G.lib().gf_assemble_jacobian_colored_f64(
+        gpu_ctx.h, plan, G.ptr(d_conn), G.ptr(d_x), G.ptr(d_u), G.ptr(d_data),
+        G.ptr(d_map), G.ptr(d_perm), G.ptr(d_k2), G.ptr(d_rc)))
+    gpu_ctx.sync()
+    G.lib().gf_asm_plan_destroy(plan)
+    kv2 = host(d_k2)
+    assert max_rel_scaled(kv2, kv_o, scale) <= 1e-12
+    assert np.abs(kv2 - kv_o).max() <= 1e-12 * np.abs(kv_o).max()
+    assert np.abs(host(d_rc) - res_o).max() <= 1e-12 * np.abs(res_o).max()
+
     # residual-only kernel
     d_r2 = torch.zeros_like(d_r)
     gpu_ctx.check(G.lib().gf_assemble_residual_mitc4_f64(
