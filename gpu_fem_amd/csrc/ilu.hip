@@ -56,52 +56,67 @@ namespace {
 
 constexpr int kMaxRowBlocks = 1024;
 
-// One 64-lane workgroup (a single wavefront, so barriers are cheap) per row
-// of the current level.
-constexpr int kIluThreads = 64;
+// One 256-thread workgroup per row of the current level.  The row's column
+// indices are staged in LDS (positions of U(k) in row i are found by binary
+// search there), L_ik is kept in LDS for the update, and the next k's
+// pattern bounds are loaded while the current update runs.
+constexpr int kIluThreads = 256;
 template <typename T>
 __global__ void __launch_bounds__(kIluThreads)
     ilu_row_kernel(const int* __restrict__ level_rows, int nrows,
                    const int* __restrict__ rowp, const int* __restrict__ cols,
                    const int* __restrict__ diag, T* vals, int* pivot) {
+  __shared__ int scol[kMaxRowBlocks];
   __shared__ int pos[kMaxRowBlocks];
+  __shared__ T lik[36];
   const int li = blockIdx.x;
   if (li >= nrows) return;
   const int i = level_rows[li];
   const int tid = threadIdx.x;
   const int r0 = rowp[i], r1 = rowp[i + 1], di = diag[i];
+  for (int t = tid; t < r1 - r0; t += blockDim.x) scol[t] = cols[r0 + t];
+  __syncthreads();
+  int kn = r0 < di ? scol[0] : 0;  // next k and its U-part bounds
+  int dn = r0 < di ? diag[kn] : 0, un = r0 < di ? rowp[kn + 1] : 0;
   for (int kk = r0; kk < di; kk++) {
-    const int k = cols[kk];
+    const int dk = dn, u0 = dn + 1, u1 = un;
+    if (kk + 1 < di) {  // bounds of the next k, loaded ahead
+      kn = scol[kk + 1 - r0];
+      dn = diag[kn];
+      un = rowp[kn + 1];
+    }
     T* Aik = vals + 36 * (size_t)kk;
-    const T* Ukk = vals + 36 * (size_t)diag[k];
+    const T* Ukk = vals + 36 * (size_t)dk;
     // L_ik = A_ik U_kk^-1 (row r by forward substitution over columns)
     if (tid < 6) {
       const int r = tid;
       T x[6];
 #pragma unroll
       for (int c = 0; c < 6; c++) {
-        T s = Aik[6 * r + c];
+        T v = Aik[6 * r + c];
 #pragma unroll
-        for (int m = 0; m < c; m++) s -= x[m] * Ukk[6 * m + c];
-        x[c] = s / Ukk[6 * c + c];
+        for (int m = 0; m < c; m++) v -= x[m] * Ukk[6 * m + c];
+        x[c] = v / Ukk[6 * c + c];
       }
 #pragma unroll
-      for (int c = 0; c < 6; c++) Aik[6 * r + c] = x[c];
+      for (int c = 0; c < 6; c++) {
+        Aik[6 * r + c] = x[c];
+        lik[6 * r + c] = x[c];
+      }
     }
     // positions in row i of the U part of row k
-    const int u0 = diag[k] + 1, u1 = rowp[k + 1];
     const int nu = u1 - u0;
     for (int t = tid; t < nu; t += blockDim.x) {
       const int j = cols[u0 + t];
-      int lo = kk + 1, hi = r1;
+      int lo = kk + 1 - r0, hi = r1 - r0;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (cols[mid] < j)
+        if (scol[mid] < j)
           lo = mid + 1;
         else
           hi = mid;
       }
-      pos[t] = (lo < r1 && cols[lo] == j) ? lo : -1;
+      pos[t] = (lo < r1 - r0 && scol[lo] == j) ? r0 + lo : -1;
     }
     __syncthreads();
     // A_ij -= L_ik U_kj
@@ -110,10 +125,10 @@ __global__ void __launch_bounds__(kIluThreads)
       const int p = pos[t];
       if (p < 0) continue;
       const T* Ukj = vals + 36 * (size_t)(u0 + t);
-      T s = 0;
+      T v = 0;
 #pragma unroll
-      for (int m = 0; m < 6; m++) s += Aik[6 * r + m] * Ukj[6 * m + c];
-      vals[36 * (size_t)p + e] -= s;
+      for (int m = 0; m < 6; m++) v += lik[6 * r + m] * Ukj[6 * m + c];
+      vals[36 * (size_t)p + e] -= v;
     }
     __syncthreads();
   }
