@@ -219,6 +219,7 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 constexpr int kSweepIter = 4;             // block batches held in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
 constexpr int kCache = 4;                 // recent rows kept in registers
+constexpr int kGather = 128;              // dependency blocks per pass
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
 // Sentinel values: signalling NaNs.  Arithmetic never produces a signalling
@@ -267,9 +268,12 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // running wavefront, or earlier in the same chunk, so the sweep cannot
 // deadlock whatever the number of resident workgroups.
 //
-// Lanes: slot t = lane / 6 (0..9) takes blocks k0+t, k0+t+10, ...; r =
-// lane % 6 is the scalar row; the 6x6 diagonal block is solved by lanes 0-5
-// with shuffles.  The last kCache rows solved by the wavefront are kept in
+// Per row: the dependencies' entries (up to kGather blocks per pass) are
+// loaded together and parked in LDS, so a row costs about two memory round
+// trips (column indices, then dependency entries) whatever its length; then
+// slot t = lane / 6 (0..9) takes blocks k0+t, k0+t+10, ... with r = lane % 6
+// the scalar row; the 6x6 diagonal block is solved by lanes 0-5 with
+// shuffles.  The last kCache rows solved by the wavefront are kept in
 // registers (the rows of a chain depend on the rows just before them).
 //
 // fill_next (lower sweep only, may be null): the entries of the row in the
@@ -284,9 +288,11 @@ __global__ void __launch_bounds__(kSweepThreads)
                          T* fill_next, unsigned long long* counter,
                          unsigned long long base, int* err) {
   using B = typename Sentinel<T>::Bits;
+  __shared__ T xs_all[kSweepThreads / 64][kGather][6];
   const int lane = threadIdx.x % 64;
   const int t = lane / 6, r = lane % 6;
   const bool act = lane < 6 * kSlots;
+  T(&xs)[kGather][6] = xs_all[threadIdx.x / 64];
   // the last kCache rows solved by this wavefront and their solutions (in
   // every lane); a chain's row usually depends on the rows just before it
   int cid[kCache];
@@ -332,68 +338,92 @@ __global__ void __launch_bounds__(kSweepThreads)
 #pragma unroll
       for (int m = 0; m < 6; m++)
         drow[m] = (lane < 6) ? lu[36 * (size_t)di + 6 * lane + m] : T(0);
+      // Per pass of up to kGather blocks: the first kSweepIter*kSlots blocks'
+      // values are loaded before anything waits; the pass's column indices
+      // are read (lane q: blocks q, q+64) and the dependencies' entries are
+      // loaded at once, polled until no sentinel is left, and parked in LDS;
+      // then lanes (t, r) accumulate A[q][r][:] . x[col_q][:].
       T acc = T(0);
-      for (int q0 = k0; q0 < k1; q0 += kSlots * kSweepIter) {
-        T a[kSweepIter][6], xv[kSweepIter][6];
-        int jj[kSweepIter];
+      for (int g0 = k0; g0 < k1; g0 += kGather) {
+        const int ng = min(kGather, k1 - g0);
+        T a[kSweepIter][6];
 #pragma unroll
         for (int it = 0; it < kSweepIter; it++) {
-          const int q = q0 + kSlots * it + t;
-          const bool ok = act && q < k1;
-          jj[it] = ok ? cols[q] : -1;
-          const T* ap = lu + 36 * (size_t)(ok ? q : 0) + 6 * r;
+          const int q = kSlots * it + t;
+          const bool ok = act && q < ng;
+          const T* ap = lu + 36 * (size_t)(g0 + (ok ? q : 0)) + 6 * r;
 #pragma unroll
           for (int c = 0; c < 6; c++) a[it][c] = ok ? ap[c] : T(0);
         }
-        // all dependency loads first (one round trip), then re-poll the
-        // entries still holding the sentinel
-        B xb[kSweepIter][6];
+        int jj[kGather / 64];
+        B xb[kGather / 64][6];
 #pragma unroll
-        for (int it = 0; it < kSweepIter; it++) {
-          const int j = jj[it];
+        for (int h = 0; h < kGather / 64; h++) {
+          const int q = lane + 64 * h;
+          jj[h] = q < ng ? cols[g0 + q] : -1;
 #pragma unroll
           for (int c = 0; c < 6; c++)
-            xb[it][c] = (j >= 0 && !cached(j)) ? load_bits(x + 6 * (size_t)j + c)
-                                                : B(0);
+            xb[h][c] = (jj[h] >= 0 && !cached(jj[h]))
+                           ? load_bits(x + 6 * (size_t)jj[h] + c)
+                           : B(0);
         }
         unsigned spins = 0;
         for (;;) {
           bool pending = false;
 #pragma unroll
-          for (int it = 0; it < kSweepIter; it++)
+          for (int h = 0; h < kGather / 64; h++)
 #pragma unroll
             for (int c = 0; c < 6; c++)
-              pending |= xb[it][c] == Sentinel<T>::value;
+              pending |= xb[h][c] == Sentinel<T>::value;
           if (!__any(pending)) break;
           if (++spins > kSpinLimit) {
             if (lane == 0)
               __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-            for (int it = 0; it < kSweepIter; it++)
+            for (int h = 0; h < kGather / 64; h++)
 #pragma unroll
-              for (int c = 0; c < 6; c++) xb[it][c] = B(0);
+              for (int c = 0; c < 6; c++) xb[h][c] = B(0);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-          for (int it = 0; it < kSweepIter; it++)
+          for (int h = 0; h < kGather / 64; h++)
 #pragma unroll
             for (int c = 0; c < 6; c++)
-              if (xb[it][c] == Sentinel<T>::value)
-                xb[it][c] = load_bits(x + 6 * (size_t)jj[it] + c);
+              if (xb[h][c] == Sentinel<T>::value)
+                xb[h][c] = load_bits(x + 6 * (size_t)jj[h] + c);
         }
+        // park the dependencies in this wavefront's LDS slice
 #pragma unroll
-        for (int it = 0; it < kSweepIter; it++)
+        for (int h = 0; h < kGather / 64; h++) {
+          const int q = lane + 64 * h;
+          if (q < ng) {
 #pragma unroll
-          for (int c = 0; c < 6; c++)
-            xv[it][c] = jj[it] >= 0 && cached(jj[it])
-                            ? cache_value(jj[it], c)
-                            : __builtin_bit_cast(T, xb[it][c]);
+            for (int c = 0; c < 6; c++)
+              xs[q][c] = cached(jj[h]) ? cache_value(jj[h], c)
+                                       : __builtin_bit_cast(T, xb[h][c]);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int it = 0; it < kSweepIter; it++)
+        for (int it = 0; it < kSweepIter; it++) {
+          const int q = kSlots * it + t;
+          if (act && q < ng) {
 #pragma unroll
-          for (int c = 0; c < 6; c++) acc += a[it][c] * xv[it][c];
+            for (int c = 0; c < 6; c++) acc += a[it][c] * xs[q][c];
+          }
+        }
+        for (int q = kSlots * kSweepIter + t; act && q < ng; q += kSlots) {
+          const T* ap = lu + 36 * (size_t)(g0 + q) + 6 * r;
+#pragma unroll
+          for (int c = 0; c < 6; c++) acc += ap[c] * xs[q][c];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
       // sum the slots of scalar row r into lane r
       T rs = T(0);
