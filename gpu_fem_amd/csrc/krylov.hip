@@ -11,6 +11,8 @@
 // pass (CGS2): two multi-dot launches and two updates per step instead of the
 // (j+1) serial dot/axpy pairs of modified Gram-Schmidt.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "common.h"
@@ -292,7 +294,8 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
   double bnorm = W.host_scal[0];
   if (bnorm == 0.0) bnorm = 1.0;
   int it = 0, outer = 0, status = GF_ERR_NO_CONVERGENCE;
-  double true_rel = 0.0;
+  double true_rel = 0.0, last_est = 0.0;
+  static const bool verbose = getenv("GPUFEM_GMRES_VERBOSE") != nullptr;
   for (;;) {
     // r = b - A x (FP64 always)
     GF_TRY(op.apply_a64(x, W.r));
@@ -303,6 +306,9 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     GF_HIP(ctx, hipStreamSynchronize(s));
     const double beta = W.host_scal[0];
     true_rel = beta / bnorm;
+    if (verbose)
+      fprintf(stderr, "gmres: cycle %d  iterations %d  true rel. residual %.3e"
+              "  last estimate %.3e\n", outer, it, true_rel, last_est / bnorm);
     if (!std::isfinite(beta)) {
       status = set_error(ctx, GF_ERR_BREAKDOWN, "GMRES: non-finite residual");
       break;
@@ -365,6 +371,7 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
         GF_HIP(ctx, hipEventSynchronize(W.ev[j % 2]));
         const double est = W.host_scal[2 * (j % 2)];
         const double hn = W.host_scal[2 * (j % 2) + 1];
+        last_est = est;
         j++;
         if (!std::isfinite(est) || est <= tol_abs || hn == 0.0 || j == limit)
           stop = true;
