@@ -497,11 +497,15 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
   // chunks: runs of rows of one level, sized so that a level still spreads
   // over the resident wavefronts (one row per chunk for small levels)
   const int target_waves = 16 * std::max(ctx->num_cus, 1);
+  const char* cenv = getenv("GPUFEM_TRSV_MINCHUNK");
+  const int min_chunk = cenv ? std::max(1, atoi(cenv)) : 1;
   auto chunks = [&](const std::vector<int>& lp, std::vector<int>& out) {
     out.assign(1, 0);
     for (size_t l = 0; l + 1 < lp.size(); l++) {
       const int cnt = lp[l + 1] - lp[l];
-      const int csz = std::max(1, std::min(16, cnt / target_waves));
+      const int csz =
+          std::max(std::min(min_chunk, (cnt + 63) / 64),
+                   std::max(1, std::min(16, cnt / target_waves)));
       for (int a = lp[l]; a < lp[l + 1]; a += csz)
         out.push_back(std::min(a + csz, lp[l + 1]));
     }
