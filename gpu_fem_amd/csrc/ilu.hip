@@ -216,9 +216,9 @@ __global__ void __launch_bounds__(256)
 
 
 constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
-constexpr int kSweepIter = 4;             // block batches held in registers
+constexpr int kSweepIter = 2;             // block batches held in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
-constexpr int kCache = 4;                 // recent rows kept in registers
+constexpr int kCache = 1;                 // recent rows kept in registers
 constexpr int kGather = 128;              // dependency blocks per pass
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
@@ -280,7 +280,7 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // upper sweep's output are set to the sentinel once the row's rhs is read,
 // which allows that output to alias the rhs.
 template <typename T, bool kLower>
-__global__ void __launch_bounds__(kSweepThreads)
+__global__ void __launch_bounds__(kSweepThreads, 4)
     trsv_syncfree_kernel(const int4* __restrict__ meta,
                          const int* __restrict__ cols,
                          const int* __restrict__ chunk_ptr, int nchunks,
