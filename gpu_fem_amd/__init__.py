@@ -85,6 +85,7 @@ _SIGS = {
     "gf_bsr_spmv_f64": [_P, _I, _P, _P, _P, _P, _P],
     "gf_bsr_spmv_f32": [_P, _I, _P, _P, _P, _P, _P],
     "gf_convert_f64_f32": [_P, ctypes.c_size_t, _P, _P],
+    "gf_bsr_resid_comp_f64": [_P, _I, _P, _P, _P, _P, _P, _P],
     "gf_bsr_resid_f64": [_P, _I, _P, _P, _P, _P, _P,
                          ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(ctypes.c_double)],

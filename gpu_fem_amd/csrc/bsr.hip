@@ -199,6 +199,43 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// r = b - A x with compensated accumulation (Ogita, Rump and Oishi,
+// "Accurate sum and dot product", SIAM J. Sci. Comput. 26(6), 2005: error-free
+// TwoProduct via FMA and TwoSum): the result is as accurate as if computed in
+// twice the working precision and then rounded.  On thin shells |A||x| is
+// ~1e9 |b|, so the plain FP64 residual has a rounding floor near 1e-8
+// relative; the outer loop of GMRES uses this one.
+__global__ void __launch_bounds__(256)
+    resid_comp_kernel(int nnodes, const int* __restrict__ rowp,
+                      const int* __restrict__ cols,
+                      const double* __restrict__ vals,
+                      const double* __restrict__ x,
+                      const double* __restrict__ b, double* __restrict__ r) {
+#pragma clang fp contract(off)  // the error-free transformations need
+                                // separately rounded products and sums
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane >= 60) return;
+  const int row = (blockIdx.x * (blockDim.x / 64) + wave) * 10 + lane / 6;
+  const int rr = lane % 6;
+  if (row >= nnodes) return;
+  double s = b[6 * (size_t)row + rr], c = 0.0;
+  for (int k = rowp[row]; k < rowp[row + 1]; k++) {
+    const double* a = vals + 36 * (size_t)k + 6 * rr;
+    const double* xb = x + 6 * (size_t)cols[k];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const double p = a[m] * xb[m];
+      const double e = __builtin_fma(a[m], xb[m], -p);  // a x = p + e
+      const double t = s - p;                           // TwoSum(s, -p)
+      const double z = t - s;
+      const double err = (s - (t - z)) + (-p - z);
+      s = t;
+      c += err - e;
+    }
+  }
+  r[6 * (size_t)row + rr] = s + c;
+}
+
 __global__ void convert_kernel(size_t n, const double* __restrict__ in,
                                float* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,6 +345,18 @@ int gf_bsr_resid_f64(gf_context ctx, int nnodes, const int* rowp,
   if (max_entry) *max_entry = h[0];
   if (norm2) *norm2 = h[1];
   return GF_OK;
+}
+
+int gf_bsr_resid_comp_f64(gf_context ctx, int nnodes, const int* rowp,
+                          const int* cols, const double* vals, const double* x,
+                          const double* b, double* r) {
+  if (!ctx || nnodes <= 0 || !rowp || !cols || !vals || !x || !b || !r ||
+      r == x)
+    return gf::set_error(ctx, GF_ERR_INVALID, "resid_comp: bad args");
+  hipLaunchKernelGGL(gf::resid_comp_kernel, dim3(gf::div_up(nnodes, 40)),
+                     dim3(256), 0, ctx->stream, nnodes, rowp, cols, vals, x, b,
+                     r);
+  return gf::check_launch(ctx, "resid_comp_kernel");
 }
 
 int gf_convert_f64_f32(gf_context ctx, size_t n, const double* in,

@@ -297,9 +297,9 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
   double true_rel = 0.0, last_est = 0.0;
   static const bool verbose = getenv("GPUFEM_GMRES_VERBOSE") != nullptr;
   for (;;) {
-    // r = b - A x (FP64 always)
-    GF_TRY(op.apply_a64(x, W.r));
-    hipLaunchKernelGGL(sub_from, grid1(N), dim3(256), 0, s, N, b, W.r);
+    // r = b - A x in FP64 with compensated accumulation
+    GF_TRY(gf_bsr_resid_comp_f64(ctx, op.n, op.krowp, op.kcols, op.kvals, x, b,
+                                 W.r));
     GF_TRY(W.dots(W.r, N, 1, W.r, W.nrm, 2));
     GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.nrm, sizeof(double),
                                hipMemcpyDeviceToHost, s));

@@ -153,6 +153,12 @@ int gf_bsr_spmv_f32(gf_context ctx, int nnodes, const int* d_rowp,
                     float* d_y);
 int gf_convert_f64_f32(gf_context ctx, size_t n, const double* d_in,
                        float* d_out);
+/* r = b - A x accumulated with error-free transformations (compensated
+ * dot products), as accurate as twice-FP64 arithmetic rounded once; the true
+ * residual of the GMRES outer loop.  r may alias b but not x. */
+int gf_bsr_resid_comp_f64(gf_context ctx, int nnodes, const int* d_rowp,
+                          const int* d_cols, const double* d_vals,
+                          const double* d_x, const double* d_b, double* d_r);
 /* r = A x - b: the entry of r of largest magnitude (signed, first index on
  * ties; CUSPARSE::get_resid, include/linalg/solvers/cusparse.h:853-918) and
  * ||r||_2.  Synchronises the context stream. */

@@ -320,3 +320,47 @@ def test_residual_diagnostic(gpu_ctx):
         assert abs(mx.value - r[i]) <= 1e-12 * abs(r[i])
         assert abs(nr.value - np.linalg.norm(r)) <= 1e-12 * np.linalg.norm(r)
     assert mx.value == 3.0  # r = -b: first of the tied entries is r[5] = 3
+
+
+def test_compensated_residual(gpu_ctx):
+    """gf_bsr_resid_comp_f64 (true residual of the GMRES outer loop) against
+    the exactly rounded residual (exact rational arithmetic), on a system
+    where b = K x rounded, so r is pure cancellation and the plain FP64
+    residual is mostly rounding error."""
+    from fractions import Fraction
+
+    say("[gpu] compensated residual plate 6x5 ss")
+    m = M.plate(6, 5, bc="ss", load="sine")
+    s = oracle_problem(m, "rcm", 0)
+    n = m.nnodes
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(6 * n) * 1e-2
+    b = oracle.bsr_spmv(s["krp"], s["kcp"], s["kv"], x)
+    krp, kcp, kv = s["krp"], s["kcp"], s["kv"]
+    exact = np.zeros(6 * n)
+    mag = np.zeros(6 * n)
+    for i in range(n):
+        for r in range(6):
+            acc = Fraction(b[6 * i + r])
+            sm = 0.0
+            for k in range(krp[i], krp[i + 1]):
+                j = kcp[k]
+                for c in range(6):
+                    a = kv[36 * k + 6 * r + c]
+                    acc -= Fraction(a) * Fraction(x[6 * j + c])
+                    sm += abs(a * x[6 * j + c])
+            exact[6 * i + r] = float(acc)
+            mag[6 * i + r] = sm
+    d = {k: dev(s[k]) for k in ("krp", "kcp", "kv")}
+    d_x, d_b = dev(x), dev(b)
+    d_r = dev(np.zeros(6 * n))
+    gpu_ctx.check(G.lib().gf_bsr_resid_comp_f64(
+        gpu_ctx.h, n, G.ptr(d["krp"]), G.ptr(d["kcp"]), G.ptr(d["kv"]),
+        G.ptr(d_x), G.ptr(d_b), G.ptr(d_r)))
+    gpu_ctx.sync()
+    r = host(d_r)
+    tol = 2.0 ** -100 * mag + 2 * np.spacing(np.abs(exact))
+    assert np.all(np.abs(r - exact) <= tol)
+    # the plain FP64 residual is far less accurate here
+    plain = b - oracle.bsr_spmv(krp, kcp, kv, x)
+    assert np.abs(plain - exact).max() > 1e3 * np.abs(r - exact).max()
