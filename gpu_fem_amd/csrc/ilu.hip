@@ -216,7 +216,13 @@ __global__ void __launch_bounds__(256)
 
 
 constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
-constexpr int kSweepIter = 2;             // block batches held in registers
+#ifndef GPUFEM_SWEEP_ITER
+#define GPUFEM_SWEEP_ITER 4
+#endif
+#ifndef GPUFEM_SWEEP_MINWAVES
+#define GPUFEM_SWEEP_MINWAVES 1
+#endif
+constexpr int kSweepIter = GPUFEM_SWEEP_ITER;  // block batches in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
 constexpr int kCache = 1;                 // recent rows kept in registers
 constexpr int kGather = 128;              // dependency blocks per pass
@@ -280,7 +286,7 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // upper sweep's output are set to the sentinel once the row's rhs is read,
 // which allows that output to alias the rhs.
 template <typename T, bool kLower>
-__global__ void __launch_bounds__(kSweepThreads, 4)
+__global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
     trsv_syncfree_kernel(const int4* __restrict__ meta,
                          const int* __restrict__ cols,
                          const int* __restrict__ chunk_ptr, int nchunks,
