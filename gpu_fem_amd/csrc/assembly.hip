@@ -193,6 +193,168 @@ __global__ void __launch_bounds__(64)
 }
 
 // ---------------------------------------------------------------------------
+// Two elements per 64-lane wavefront (32 lanes each), quadrature points
+// streamed through LDS: per point the element's lanes 0-23 form column j of
+// B_q and D_q = detXd C B_q (the column's tying-strain part is formed once
+// and kept in registers), then every lane adds two 3x3 tiles of
+// B_q^T D_q.  LDS per element holds the geometry and one point's B and D
+// (6.2 KB), so a compute unit keeps about three times more elements in
+// flight than with all four points' B and D resident.  Scatter as in
+// jacobian_kernel (atomic, or colour-ordered stores / read-add-writes).
+struct ElemSlotQ {
+  double X[12];
+  double u[24];
+  double data[4];
+  mitc4::Geom g;
+  double B[9][24];
+  double D[9][24];
+};
+
+template <bool kColored>
+__global__ void __launch_bounds__(64)
+    jacobian2_kernel(int nelems, const int* __restrict__ elem_list,
+                     const int* __restrict__ first_mask,
+                     const int* __restrict__ conn,
+                     const double* __restrict__ xpts,
+                     const double* __restrict__ vars,
+                     const double* __restrict__ data,
+                     const int* __restrict__ elem_ind_map,
+                     const int* __restrict__ perm, double* kvals,
+                     double* res) {
+  __shared__ ElemSlotQ S[2];
+  __shared__ double rp[2][8][24];  // residual partials per tile column
+  const int half = threadIdx.x / 32, l = threadIdx.x % 32;
+  const int slot = blockIdx.x * 2 + half;
+  const bool valid = slot < nelems;
+  const int e = valid ? (kColored ? elem_list[slot] : slot) : 0;
+  const int mask = (kColored && valid) ? first_mask[slot] : 0;
+  ElemSlotQ& s = S[half];
+  if (valid) {
+    for (int v = l; v < 40; v += 32) {
+      if (v < 12) {
+        s.X[v] = xpts[3 * (size_t)conn[4 * (size_t)e + v / 3] + v % 3];
+      } else if (v < 36) {
+        const int i = v - 12;
+        s.u[i] = vars ? vars[6 * (size_t)conn[4 * (size_t)e + i / 6] + i % 6]
+                      : 0.0;
+      } else {
+        s.data[v - 36] = data[4 * (size_t)e + (v - 36)];
+      }
+    }
+  }
+  __syncthreads();
+  if (valid) {
+    if (l < 4)
+      mitc4::geom_node(l, s.X, s.g);
+    else if (l < 9)
+      mitc4::geom_tying_x(l - 4, s.X, s.g);
+  }
+  __syncthreads();
+  if (valid) {
+    if (l < 4)
+      mitc4::geom_quad(l, s.X, s.g);
+    else if (l < 9)
+      mitc4::geom_tying_n(l - 4, s.g);
+  }
+  __syncthreads();
+  // tiles (I, J) and (I + 4, J) of the 8 x 8 tile grid
+  const int I0 = l / 8, J = l % 8;
+  double acc[2][3][3];
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[h][r][c] = 0.0;
+  mitc4::BColumnPrep prep;
+  if (valid && l < 24) mitc4::b_column_prep(l, s.g, prep);
+#pragma unroll 1
+  for (int q = 0; q < 4; q++) {
+    if (valid && l < 24) {
+      double col[9], st[9];
+      mitc4::b_column_q(l, q, s.g, prep, col);
+      mitc4::stress(s.data, col, st);
+      const double sc = s.g.scale[q];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        s.B[k][l] = col[k];
+        s.D[k][l] = sc * st[k];
+      }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const double d0 = s.D[k][3 * J], d1 = s.D[k][3 * J + 1],
+                     d2 = s.D[k][3 * J + 2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int I = I0 + 4 * h;
+          const double b0 = s.B[k][3 * I], b1 = s.B[k][3 * I + 1],
+                       b2 = s.B[k][3 * I + 2];
+          acc[h][0][0] += b0 * d0;
+          acc[h][0][1] += b0 * d1;
+          acc[h][0][2] += b0 * d2;
+          acc[h][1][0] += b1 * d0;
+          acc[h][1][1] += b1 * d1;
+          acc[h][1][2] += b1 * d2;
+          acc[h][2][0] += b2 * d0;
+          acc[h][2][1] += b2 * d1;
+          acc[h][2][2] += b2 * d2;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (valid) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int I = I0 + 4 * h;
+      const int a = I / 2, b = J / 2, r0 = 3 * (I % 2), c0 = 3 * (J % 2);
+      double* dst = kvals + 36 * (size_t)elem_ind_map[16 * (size_t)e + 4 * a + b];
+      const bool first = (mask >> (4 * a + b)) & 1;
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          double* d = &dst[6 * (r0 + r) + c0 + c];
+          if (!kColored)
+            unsafeAtomicAdd(d, acc[h][r][c]);
+          else
+            *d = first ? acc[h][r][c] : *d + acc[h][r][c];
+        }
+    }
+  }
+  if (!res) return;  // uniform across the workgroup
+  if (valid) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int I = I0 + 4 * h;
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) v += acc[h][r][c] * s.u[3 * J + c];
+        rp[half][J][3 * I + r] = v;
+      }
+    }
+  }
+  __syncthreads();
+  if (valid && l < 24) {
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v += rp[half][j][l];
+    const int node = conn[4 * (size_t)e + l / 6];
+    const int row = perm ? perm[node] : node;
+    double* d = &res[6 * (size_t)row + l % 6];
+    if (!kColored)
+      unsafeAtomicAdd(d, v);
+    else
+      *d = ((mask >> (16 + l / 6)) & 1) ? v : *d + v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batched Jacobian: six elements per 64-lane wavefront, ten lanes per
 // element.  Lane l of an element owns one of the ten node-pair blocks
 // (a <= b) of the symmetric K_e and keeps its 6x6 sum in registers; the
@@ -422,6 +584,20 @@ __global__ void __launch_bounds__(64)
 }  // namespace
 }  // namespace gf
 
+namespace {
+// GPUFEM_ASSEMBLY=element selects the one-element-per-wavefront kernel,
+// =batched the six-elements-per-wavefront one (atomic entry point only)
+int assembly_variant() {
+  static const int v = [] {
+    const char* e = getenv("GPUFEM_ASSEMBLY");
+    if (e && std::string(e) == "element") return 1;
+    if (e && std::string(e) == "batched") return 2;
+    return 0;
+  }();
+  return v;
+}
+}  // namespace
+
 extern "C" {
 
 int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
@@ -433,16 +609,19 @@ int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
       !elem_ind_map || !kvals)
     return gf::set_error(ctx, GF_ERR_INVALID, "assemble_jacobian: bad args");
   if (nelems == 0) return GF_OK;
-  static const bool batched = [] {
-    const char* v = getenv("GPUFEM_ASSEMBLY");
-    return v && std::string(v) == "batched";
-  }();
-  if (!batched) {
+  if (assembly_variant() == 1) {
     hipLaunchKernelGGL(gf::jacobian_kernel<false>, dim3(nelems), dim3(64), 0,
                        ctx->stream, nelems, (const int*)nullptr,
                        (const int*)nullptr, conn, xpts, vars, data,
                        elem_ind_map, perm, kvals, res);
     return gf::check_launch(ctx, "jacobian_kernel");
+  }
+  if (assembly_variant() == 0) {
+    hipLaunchKernelGGL(gf::jacobian2_kernel<false>,
+                       dim3(gf::div_up(nelems, 2)), dim3(64), 0, ctx->stream,
+                       nelems, (const int*)nullptr, (const int*)nullptr, conn,
+                       xpts, vars, data, elem_ind_map, perm, kvals, res);
+    return gf::check_launch(ctx, "jacobian2_kernel");
   }
   hipLaunchKernelGGL(gf::jacobian_batched_kernel,
                      dim3(gf::div_up(nelems, gf::kElemsPerWave)), dim3(64), 0,
@@ -450,6 +629,7 @@ int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
                      perm, kvals, res);
   return gf::check_launch(ctx, "jacobian_batched_kernel");
 }
+
 
 int gf_asm_plan_create(gf_context ctx, int nelems, int nnodes,
                        const int* conn, const int* elem_ind_map, int nnzb,
@@ -535,9 +715,16 @@ int gf_assemble_jacobian_colored_f64(gf_context ctx, gf_asm_plan pl,
                                ctx->stream));
   for (int c = 0; c < pl->ncolors; c++) {
     const int c0 = pl->color_ptr[c], cnt = pl->color_ptr[c + 1] - c0;
-    hipLaunchKernelGGL(gf::jacobian_kernel<true>, dim3(cnt), dim3(64), 0,
-                       ctx->stream, cnt, pl->elems + c0, pl->mask + c0, conn,
-                       xpts, vars, data, elem_ind_map, perm, kvals, res);
+    if (assembly_variant() == 1)
+      hipLaunchKernelGGL(gf::jacobian_kernel<true>, dim3(cnt), dim3(64), 0,
+                         ctx->stream, cnt, pl->elems + c0, pl->mask + c0,
+                         conn, xpts, vars, data, elem_ind_map, perm, kvals,
+                         res);
+    else
+      hipLaunchKernelGGL(gf::jacobian2_kernel<true>, dim3(gf::div_up(cnt, 2)),
+                         dim3(64), 0, ctx->stream, cnt, pl->elems + c0,
+                         pl->mask + c0, conn, xpts, vars, data, elem_ind_map,
+                         perm, kvals, res);
   }
   return gf::check_launch(ctx, "jacobian_kernel<colored>");
 }
