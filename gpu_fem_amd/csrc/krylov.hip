@@ -21,11 +21,13 @@
 namespace gf {
 namespace {
 
-constexpr int kDotBlocks = 512;
+constexpr int kDotBlocks = 256;  // slices of the index range
 constexpr int kDotThreads = 256;
 constexpr int kDotChunk = 8;
 
-// partial[b * nvec + i] = sum over block b's slice of V_i . w
+// partial[b * nvec + i] = sum over slice b of V_i . w.  Grid: x = slice of
+// the index range (kDotBlocks slices), y = chunk of kDotChunk vectors, so a
+// long basis spreads over kDotBlocks x ceil(nvec / kDotChunk) workgroups.
 __global__ void __launch_bounds__(kDotThreads)
     multi_dot_partial(size_t N, int nvec, const double* __restrict__ V,
                       size_t ldv, const double* __restrict__ w,
@@ -34,30 +36,30 @@ __global__ void __launch_bounds__(kDotThreads)
   __shared__ double red[kDotThreads / 64][kDotChunk];
   if (gate && *gate == 0) return;
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  for (int i0 = 0; i0 < nvec; i0 += kDotChunk) {
-    double acc[kDotChunk];
+  const int i0 = blockIdx.y * kDotChunk;
+  const size_t len = (N + gridDim.x - 1) / gridDim.x;
+  const size_t t0 = (size_t)blockIdx.x * len;
+  const size_t t1 = t0 + len < N ? t0 + len : N;
+  double acc[kDotChunk];
 #pragma unroll
-    for (int c = 0; c < kDotChunk; c++) acc[c] = 0.0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
-         t += (size_t)gridDim.x * blockDim.x) {
-      const double wt = w[t];
+  for (int c = 0; c < kDotChunk; c++) acc[c] = 0.0;
+  for (size_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const double wt = w[t];
 #pragma unroll
-      for (int c = 0; c < kDotChunk; c++)
-        if (i0 + c < nvec) acc[c] += V[(size_t)(i0 + c) * ldv + t] * wt;
-    }
+    for (int c = 0; c < kDotChunk; c++)
+      if (i0 + c < nvec) acc[c] += V[(size_t)(i0 + c) * ldv + t] * wt;
+  }
 #pragma unroll
-    for (int c = 0; c < kDotChunk; c++) {
-      double v = acc[c];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-      if (lane == 0) red[wave][c] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < kDotChunk && i0 + (int)threadIdx.x < nvec) {
-      double v = 0.0;
-      for (int wv = 0; wv < kDotThreads / 64; wv++) v += red[wv][threadIdx.x];
-      partial[(size_t)blockIdx.x * nvec + i0 + threadIdx.x] = v;
-    }
-    __syncthreads();
+  for (int c = 0; c < kDotChunk; c++) {
+    double v = acc[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kDotChunk && i0 + (int)threadIdx.x < nvec) {
+    double v = 0.0;
+    for (int wv = 0; wv < kDotThreads / 64; wv++) v += red[wv][threadIdx.x];
+    partial[(size_t)blockIdx.x * nvec + i0 + threadIdx.x] = v;
   }
 }
 
@@ -160,31 +162,50 @@ __global__ void to_f64(size_t N, const float* __restrict__ in,
 // Arnoldi column j: H[0..j, j] = h, H[j+1, j] = hn; apply the previous
 // rotations, form the new one, update g; out = |g[j+1]|, g_j1 stored.
 // State layout in `st`: H (m+1)*m | cs m | sn m | g m+1
-__global__ void givens_step(int j, int m, const double* __restrict__ h,
-                            const double* __restrict__ hn,
-                            double* __restrict__ st, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Column j of the Hessenberg matrix: store h, apply the previous rotations
+// (a sequential recurrence, run by one lane out of LDS after the rotations
+// and the column are staged there by the whole workgroup), form rotation j
+// and update g.
+__global__ void __launch_bounds__(256)
+    givens_step(int j, int m, const double* __restrict__ h,
+                const double* __restrict__ hn, double* __restrict__ st,
+                double* out) {
+  extern __shared__ double sh[];  // col[m + 1], cs[m], sn[m]
+  double* col = sh;
+  double* lcs = sh + (m + 1);
+  double* lsn = lcs + m;
   double* H = st;
   double* cs = st + (size_t)(m + 1) * m;
   double* sn = cs + m;
   double* g = sn + m;
-  for (int i = 0; i <= j; i++) H[i * m + j] = h[i];
-  H[(j + 1) * m + j] = *hn;
-  for (int i = 0; i < j; i++) {
-    const double a = H[i * m + j], c = H[(i + 1) * m + j];
-    H[i * m + j] = cs[i] * a + sn[i] * c;
-    H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+  for (int i = threadIdx.x; i <= j; i += blockDim.x) col[i] = h[i];
+  for (int i = threadIdx.x; i < j; i += blockDim.x) {
+    lcs[i] = cs[i];
+    lsn[i] = sn[i];
   }
-  const double a = H[j * m + j], c = H[(j + 1) * m + j];
-  const double den = sqrt(a * a + c * c);
-  cs[j] = den > 0.0 ? a / den : 1.0;
-  sn[j] = den > 0.0 ? c / den : 0.0;
-  H[j * m + j] = cs[j] * a + sn[j] * c;
-  H[(j + 1) * m + j] = 0.0;
-  g[j + 1] = -sn[j] * g[j];
-  g[j] = cs[j] * g[j];
-  out[0] = fabs(g[j + 1]);
-  out[1] = *hn;
+  if (threadIdx.x == 0) col[j + 1] = *hn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < j; i++) {
+      const double a = col[i], c = col[i + 1];
+      col[i] = lcs[i] * a + lsn[i] * c;
+      col[i + 1] = -lsn[i] * a + lcs[i] * c;
+    }
+    const double a = col[j], c = col[j + 1];
+    const double den = sqrt(a * a + c * c);
+    const double cj = den > 0.0 ? a / den : 1.0;
+    const double sj = den > 0.0 ? c / den : 0.0;
+    cs[j] = cj;
+    sn[j] = sj;
+    col[j] = cj * a + sj * c;
+    col[j + 1] = 0.0;
+    g[j + 1] = -sj * g[j];
+    g[j] = cj * g[j];
+    out[0] = fabs(g[j + 1]);
+    out[1] = *hn;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i <= j + 1; i += blockDim.x) H[i * m + j] = col[i];
 }
 
 __global__ void init_g(int m, const double* __restrict__ beta,
@@ -194,16 +215,22 @@ __global__ void init_g(int m, const double* __restrict__ beta,
   if (i <= m) g[i] = (i == 0) ? *beta : 0.0;
 }
 
-// y = H[0:k,0:k]^-1 g[0:k]
-__global__ void solve_upper(int k, int m, const double* __restrict__ st,
-                            double* __restrict__ y) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// y = H[0:k,0:k]^-1 g[0:k] by column-oriented back substitution: one lane
+// forms y_i, then the workgroup removes column i from the right-hand side
+__global__ void __launch_bounds__(1024)
+    solve_upper(int k, int m, const double* __restrict__ st,
+                double* __restrict__ y) {
+  extern __shared__ double rhs[];  // k entries
   const double* H = st;
   const double* g = st + (size_t)(m + 1) * m + 2 * m;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) rhs[i] = g[i];
+  __syncthreads();
   for (int i = k - 1; i >= 0; i--) {
-    double s = g[i];
-    for (int t = i + 1; t < k; t++) s -= H[i * m + t] * y[t];
-    y[i] = s / H[i * m + i];
+    const double yi = rhs[i] / H[(size_t)i * m + i];
+    for (int t = threadIdx.x; t < i; t += blockDim.x)
+      rhs[t] -= H[(size_t)t * m + i] * yi;
+    if (threadIdx.x == 0) y[i] = yi;
+    __syncthreads();
   }
 }
 
@@ -271,8 +298,10 @@ void KrylovWork::release() {
 // out[0..nvec) = V_i . w  (mode: 0 store, 1 add, 2 sqrt)
 int KrylovWork::dots(const double* Vb, size_t ldv, int nvec, const double* x,
                      double* out, int mode, const int* gate) {
-  hipLaunchKernelGGL(multi_dot_partial, dim3(kDotBlocks), dim3(kDotThreads), 0,
-                     ctx->stream, N, nvec, Vb, ldv, x, partial, gate);
+  hipLaunchKernelGGL(multi_dot_partial,
+                     dim3(kDotBlocks, div_up(nvec, kDotChunk)),
+                     dim3(kDotThreads), 0, ctx->stream, N, nvec, Vb, ldv, x,
+                     partial, gate);
   hipLaunchKernelGGL(multi_dot_final, dim3(nvec), dim3(kDotThreads), 0,
                      ctx->stream, (int)kDotBlocks, nvec, partial, out, mode,
                      gate);
@@ -357,7 +386,8 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
         GF_TRY(W.dots(W.w, N, 1, W.w, W.nrm, 2, W.gate));
         hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.w,
                            W.nrm, W.V + (size_t)(js + 1) * N);
-        hipLaunchKernelGGL(givens_step, dim3(1), dim3(64), 0, s, js, m, W.h,
+        hipLaunchKernelGGL(givens_step, dim3(1), dim3(256),
+                           sizeof(double) * (3 * (size_t)m + 1), s, js, m, W.h,
                            W.nrm, W.state, W.scal);
         GF_TRY(check_launch(ctx, "arnoldi step"));
         GF_HIP(ctx, hipMemcpyAsync(W.host_scal + 2 * (js % 2), W.scal,
@@ -380,8 +410,9 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     it += j;
     GF_HIP(ctx, hipStreamSynchronize(s));
     // x += M^-1 V y
-    hipLaunchKernelGGL(solve_upper, dim3(1), dim3(64), 0, s, j, m, W.state,
-                       W.y);
+    hipLaunchKernelGGL(solve_upper, dim3(1), dim3(1024),
+                       sizeof(double) * (size_t)(j > 0 ? j : 1), s, j, m,
+                       W.state, W.y);
     hipLaunchKernelGGL(lincomb, grid1(N), dim3(256), 0, s, N, j, W.V, N, W.y,
                        W.tmp);
     GF_TRY(op.apply_prec(W.tmp, W.z, prec));
