@@ -222,6 +222,12 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 #ifndef GPUFEM_SWEEP_MINWAVES
 #define GPUFEM_SWEEP_MINWAVES 1
 #endif
+#ifndef GPUFEM_SWEEP_BACKOFF
+#define GPUFEM_SWEEP_BACKOFF 1  // longest sleep between polls, x 64 cycles
+#endif
+#ifndef GPUFEM_SWEEP_WAVES_PER_CU
+#define GPUFEM_SWEEP_WAVES_PER_CU 16
+#endif
 constexpr int kSweepIter = GPUFEM_SWEEP_ITER;  // block batches in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
 constexpr int kCache = 1;                 // recent rows kept in registers
@@ -374,6 +380,7 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
                            : B(0);
         }
         unsigned spins = 0;
+        int nap = 1;
         for (;;) {
           bool pending = false;
 #pragma unroll
@@ -392,7 +399,10 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
               for (int c = 0; c < 6; c++) xb[h][c] = B(0);
             break;
           }
-          __builtin_amdgcn_s_sleep(1);
+          // back off while the dependencies are not ready: thousands of
+          // waiting wavefronts polling at full rate load the memory system
+          for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(1);
+          nap = nap * 2 > GPUFEM_SWEEP_BACKOFF ? GPUFEM_SWEEP_BACKOFF : nap * 2;
 #pragma unroll
           for (int h = 0; h < kGather / 64; h++)
 #pragma unroll
@@ -502,7 +512,8 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
   gf::level_schedule(n, rowp, cols, false, h->uptr, urows);
   // chunks: runs of rows of one level, sized so that a level still spreads
   // over the resident wavefronts (one row per chunk for small levels)
-  const int target_waves = 16 * std::max(ctx->num_cus, 1);
+  const int target_waves =
+      GPUFEM_SWEEP_WAVES_PER_CU * std::max(ctx->num_cus, 1);
   const char* cenv = getenv("GPUFEM_TRSV_MINCHUNK");
   const int min_chunk = cenv ? std::max(1, atoi(cenv)) : 1;
   auto chunks = [&](const std::vector<int>& lp, std::vector<int>& out) {
