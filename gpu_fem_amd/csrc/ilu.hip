@@ -234,7 +234,6 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 #endif
 constexpr int kSweepIter = GPUFEM_SWEEP_ITER;  // block batches in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
-constexpr int kCache = 1;                 // recent rows kept in registers
 constexpr int kGather = 128;              // dependency blocks per pass
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
@@ -295,7 +294,7 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // fill_next (lower sweep only, may be null): the entries of the row in the
 // upper sweep's output are set to the sentinel once the row's rhs is read,
 // which allows that output to alias the rhs.
-template <typename T, bool kLower>
+template <typename T, bool kLower, int kRows>
 __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
     trsv_syncfree_kernel(const int4* __restrict__ meta,
                          const int* __restrict__ cols,
@@ -304,34 +303,23 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
                          T* fill_next, unsigned long long* counter,
                          unsigned long long base, int* err) {
   using B = typename Sentinel<T>::Bits;
-  __shared__ T xs_all[kSweepThreads / 64][kGather][6];
+  constexpr int G = 64 / kRows;        // lanes per row
+  constexpr int kS = G / 6;            // block slots per row (10 or 5)
+  constexpr int kH = kGather / G;      // dependency blocks per lane per pass
+  constexpr bool kUseCache = kRows == 1;
+  __shared__ T xs_all[kSweepThreads / 64][kRows][kGather][6];
   const int lane = threadIdx.x % 64;
-  const int t = lane / 6, r = lane % 6;
-  const bool act = lane < 6 * kSlots;
-  T(&xs)[kGather][6] = xs_all[threadIdx.x / 64];
-  // the last kCache rows solved by this wavefront and their solutions (in
-  // every lane); a chain's row usually depends on the rows just before it
-  int cid[kCache];
-  T cx[kCache][6];
+  const int grp = lane / G, gl = lane % G, g0 = grp * G;
+  const int t = gl / 6, r = gl % 6;
+  const bool act = gl < 6 * kS;
+  T(&xs)[kGather][6] = xs_all[threadIdx.x / 64][grp];
+  // the last row solved by this wavefront (one row per wavefront only): a
+  // chain's row usually depends on the row just before it
+  int cid = -1;
+  T cx[6];
 #pragma unroll
-  for (int d = 0; d < kCache; d++) {
-    cid[d] = -1;
-#pragma unroll
-    for (int c = 0; c < 6; c++) cx[d][c] = T(0);
-  }
-  auto cached = [&](int j) {
-    bool hit = false;
-#pragma unroll
-    for (int d = 0; d < kCache; d++) hit |= cid[d] == j;
-    return hit;
-  };
-  auto cache_value = [&](int j, int c) {
-    T v = T(0);
-#pragma unroll
-    for (int d = 0; d < kCache; d++)
-      if (cid[d] == j) v = cx[d][c];
-    return v;
-  };
+  for (int c = 0; c < 6; c++) cx[c] = T(0);
+  auto cached = [&](int j) { return kUseCache && cid == j; };
   unsigned long long tk = 0;
   if (lane == 0) tk = atomicAdd(counter, 1ull);
   tk = __shfl(tk, 0, 64);
@@ -341,42 +329,51 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
     unsigned long long tn = 0;
     if (lane == 0) tn = atomicAdd(counter, 1ull);
     const int p0 = chunk_ptr[kc], p1 = chunk_ptr[kc + 1];
-    int4 mnext = meta[p0];
-    for (int p = p0; p < p1; p++) {
+    int4 mnext = (p0 + grp < p1) ? meta[p0 + grp] : make_int4(0, 0, 0, 0);
+    for (int pb = p0; pb < p1; pb += kRows) {
+      // rows pb + grp of one level (independent when kRows > 1)
+      const bool live = pb + grp < p1;
       const int4 m = mnext;
-      if (p + 1 < p1) mnext = meta[p + 1];
-      const int i = m.x, k0 = m.y, k1 = m.z, di = m.w;
-      T s = (lane < 6) ? b[6 * (size_t)i + lane] : T(0);
-      if (kLower && fill_next && lane < 6)
-        reinterpret_cast<B*>(fill_next)[6 * (size_t)i + lane] =
+      if (pb + kRows + grp < p1) mnext = meta[pb + kRows + grp];
+      const int i = m.x, k0 = live ? m.y : 0, k1 = live ? m.z : 0, di = m.w;
+      T s = (live && gl < 6) ? b[6 * (size_t)i + gl] : T(0);
+      if (kLower && fill_next && live && gl < 6)
+        reinterpret_cast<B*>(fill_next)[6 * (size_t)i + gl] =
             Sentinel<T>::value;
       T drow[6];
 #pragma unroll
-      for (int m = 0; m < 6; m++)
-        drow[m] = (lane < 6) ? lu[36 * (size_t)di + 6 * lane + m] : T(0);
-      // Per pass of up to kGather blocks: the first kSweepIter*kSlots blocks'
+      for (int mm = 0; mm < 6; mm++)
+        drow[mm] = (live && gl < 6) ? lu[36 * (size_t)di + 6 * gl + mm] : T(0);
+      // Per pass of up to kGather blocks: the first kSweepIter*kS blocks'
       // values are loaded before anything waits; the pass's column indices
-      // are read (lane q: blocks q, q+64) and the dependencies' entries are
-      // loaded at once, polled until no sentinel is left, and parked in LDS;
-      // then lanes (t, r) accumulate A[q][r][:] . x[col_q][:].
+      // are read (lane q: blocks q, q+G, ...) and the dependencies' entries
+      // are loaded at once, polled until no sentinel is left, and parked in
+      // LDS; then lanes (t, r) accumulate A[q][r][:] . x[col_q][:].
       T acc = T(0);
-      for (int g0 = k0; g0 < k1; g0 += kGather) {
-        const int ng = min(kGather, k1 - g0);
+      const int nb = k1 - k0;
+      int npass = (nb + kGather - 1) / kGather;
+      // every row group of the wavefront runs the same number of passes
+#pragma unroll
+      for (int o = G; o < 64; o <<= 1)
+        npass = max(npass, __shfl_xor(npass, o, 64));
+      for (int ps = 0; ps < npass; ps++) {
+        const int g0b = k0 + ps * kGather;
+        const int ng = max(0, min(kGather, k1 - g0b));
         T a[kSweepIter][6];
 #pragma unroll
         for (int it = 0; it < kSweepIter; it++) {
-          const int q = kSlots * it + t;
+          const int q = kS * it + t;
           const bool ok = act && q < ng;
-          const T* ap = lu + 36 * (size_t)(g0 + (ok ? q : 0)) + 6 * r;
+          const T* ap = lu + 36 * (size_t)(ok ? g0b + q : 0) + 6 * r;
 #pragma unroll
           for (int c = 0; c < 6; c++) a[it][c] = ok ? ap[c] : T(0);
         }
-        int jj[kGather / 64];
-        B xb[kGather / 64][6];
+        int jj[kH];
+        B xb[kH][6];
 #pragma unroll
-        for (int h = 0; h < kGather / 64; h++) {
-          const int q = lane + 64 * h;
-          jj[h] = q < ng ? cols[g0 + q] : -1;
+        for (int h = 0; h < kH; h++) {
+          const int q = gl + G * h;
+          jj[h] = q < ng ? cols[g0b + q] : -1;
 #pragma unroll
           for (int c = 0; c < 6; c++)
             xb[h][c] = (jj[h] >= 0 && !cached(jj[h]))
@@ -388,7 +385,7 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
         for (;;) {
           bool pending = false;
 #pragma unroll
-          for (int h = 0; h < kGather / 64; h++)
+          for (int h = 0; h < kH; h++)
 #pragma unroll
             for (int c = 0; c < 6; c++)
               pending |= xb[h][c] == Sentinel<T>::value;
@@ -398,31 +395,30 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
               __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-            for (int h = 0; h < kGather / 64; h++)
+            for (int h = 0; h < kH; h++)
 #pragma unroll
               for (int c = 0; c < 6; c++) xb[h][c] = B(0);
             break;
           }
-          // back off while the dependencies are not ready: thousands of
-          // waiting wavefronts polling at full rate load the memory system
+          // back off while the dependencies are not ready: many waiting
+          // wavefronts polling at full rate load the memory system
           for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(1);
           nap = nap * 2 > GPUFEM_SWEEP_BACKOFF ? GPUFEM_SWEEP_BACKOFF : nap * 2;
 #pragma unroll
-          for (int h = 0; h < kGather / 64; h++)
+          for (int h = 0; h < kH; h++)
 #pragma unroll
             for (int c = 0; c < 6; c++)
               if (xb[h][c] == Sentinel<T>::value)
                 xb[h][c] = load_bits(x + 6 * (size_t)jj[h] + c);
         }
-        // park the dependencies in this wavefront's LDS slice
+        // park the dependencies in this row group's LDS slice
 #pragma unroll
-        for (int h = 0; h < kGather / 64; h++) {
-          const int q = lane + 64 * h;
+        for (int h = 0; h < kH; h++) {
+          const int q = gl + G * h;
           if (q < ng) {
 #pragma unroll
             for (int c = 0; c < 6; c++)
-              xs[q][c] = cached(jj[h]) ? cache_value(jj[h], c)
-                                       : __builtin_bit_cast(T, xb[h][c]);
+              xs[q][c] = cached(jj[h]) ? cx[c] : __builtin_bit_cast(T, xb[h][c]);
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -430,14 +426,14 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int it = 0; it < kSweepIter; it++) {
-          const int q = kSlots * it + t;
+          const int q = kS * it + t;
           if (act && q < ng) {
 #pragma unroll
             for (int c = 0; c < 6; c++) acc += a[it][c] * xs[q][c];
           }
         }
-        for (int q = kSlots * kSweepIter + t; act && q < ng; q += kSlots) {
-          const T* ap = lu + 36 * (size_t)(g0 + q) + 6 * r;
+        for (int q = kS * kSweepIter + t; act && q < ng; q += kS) {
+          const T* ap = lu + 36 * (size_t)(g0b + q) + 6 * r;
 #pragma unroll
           for (int c = 0; c < 6; c++) acc += ap[c] * xs[q][c];
         }
@@ -445,40 +441,37 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      // sum the slots of scalar row r into lane r
+      // sum the slots of scalar row r into lane r of the group
       T rs = T(0);
 #pragma unroll
-      for (int u = 0; u < kSlots; u++) rs += __shfl(acc, 6 * u + (lane % 6), 64);
-      s -= (lane < 6) ? rs : T(0);
+      for (int u = 0; u < kS; u++) rs += __shfl(acc, g0 + 6 * u + (gl % 6), 64);
+      s -= (gl < 6) ? rs : T(0);
       if (kLower) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) {
-          const T ym = __shfl(s, m, 64);
-          if (lane < 6 && lane > m) s -= drow[m] * ym;
+        for (int mm = 0; mm < 5; mm++) {
+          const T ym = __shfl(s, g0 + mm, 64);
+          if (gl < 6 && gl > mm) s -= drow[mm] * ym;
         }
       } else {
 #pragma unroll
-        for (int m = 5; m >= 0; m--) {
-          if (lane == m) s /= drow[m];
-          const T ym = __shfl(s, m, 64);
-          if (lane < 6 && lane < m) s -= drow[m] * ym;
+        for (int mm = 5; mm >= 0; mm--) {
+          if (gl == mm) s /= drow[mm];
+          const T ym = __shfl(s, g0 + mm, 64);
+          if (gl < 6 && gl < mm) s -= drow[mm] * ym;
         }
       }
-      if (lane < 6) store_value(x + 6 * (size_t)i + lane, s);
+      if (live && gl < 6) store_value(x + 6 * (size_t)i + gl, s);
+      if (kUseCache) {
+        cid = i;
 #pragma unroll
-      for (int d = kCache - 1; d > 0; d--) {
-        cid[d] = cid[d - 1];
-#pragma unroll
-        for (int c = 0; c < 6; c++) cx[d][c] = cx[d - 1][c];
+        for (int c = 0; c < 6; c++) cx[c] = __shfl(s, c, 64);
       }
-      cid[0] = i;
-#pragma unroll
-      for (int c = 0; c < 6; c++) cx[0][c] = __shfl(s, c, 64);
     }
     tn = __shfl(tn, 0, 64);
     kc = (long long)(tn - base);
   }
 }
+
 }  // namespace
 }  // namespace gf
 
@@ -704,15 +697,27 @@ int ilu_apply(gf_ilu h, const T* lu, const T* b, T* tmp, T* x) {
     const size_t N = 6 * (size_t)h->n;
     hipLaunchKernelGGL(gf::fill_sentinel_kernel<T>, dim3(gf::div_up(N, 256)),
                        dim3(256), 0, ctx->stream, N, tmp);
-    hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->lmeta,
-                       h->cols, h->lchunk, h->nlchunk, lu, b, tmp, x,
-                       h->counter, h->base, h->err);
+    if (h->lchains)
+      hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true, 1>),
+                         dim3(h->grid), dim3(gf::kSweepThreads), 0,
+                         ctx->stream, h->lmeta, h->cols, h->lchunk, h->nlchunk,
+                         lu, b, tmp, x, h->counter, h->base, h->err);
+    else
+      hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, true, 2>),
+                         dim3(h->grid), dim3(gf::kSweepThreads), 0,
+                         ctx->stream, h->lmeta, h->cols, h->lchunk, h->nlchunk,
+                         lu, b, tmp, x, h->counter, h->base, h->err);
     h->base += (unsigned long long)h->nlchunk + waves;
-    hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false>), dim3(h->grid),
-                       dim3(gf::kSweepThreads), 0, ctx->stream, h->umeta,
-                       h->cols, h->uchunk, h->nuchunk, lu, tmp, x, (T*)nullptr,
-                       h->counter, h->base, h->err);
+    if (h->uchains)
+      hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false, 1>),
+                         dim3(h->grid), dim3(gf::kSweepThreads), 0,
+                         ctx->stream, h->umeta, h->cols, h->uchunk, h->nuchunk,
+                         lu, tmp, x, (T*)nullptr, h->counter, h->base, h->err);
+    else
+      hipLaunchKernelGGL((gf::trsv_syncfree_kernel<T, false, 2>),
+                         dim3(h->grid), dim3(gf::kSweepThreads), 0,
+                         ctx->stream, h->umeta, h->cols, h->uchunk, h->nuchunk,
+                         lu, tmp, x, (T*)nullptr, h->counter, h->base, h->err);
     h->base += (unsigned long long)h->nuchunk + waves;
     return gf::check_launch(ctx, "trsv_syncfree_kernel");
   }
