@@ -67,7 +67,9 @@ __global__ void __launch_bounds__(kIluThreads)
                    const int* __restrict__ rowp, const int* __restrict__ cols,
                    const int* __restrict__ diag, T* vals, int* pivot) {
   __shared__ int scol[kMaxRowBlocks];
-  __shared__ int pos[kMaxRowBlocks];
+  __shared__ int pos[2][kMaxRowBlocks];  // positions for this k and the next
+  __shared__ int ub[2][2];               // U-part bounds [u0, u1) per buffer
+  __shared__ T udg[2][36];               // U_kk per buffer
   __shared__ T lik[36];
   const int li = blockIdx.x;
   if (li >= nrows) return;
@@ -76,18 +78,38 @@ __global__ void __launch_bounds__(kIluThreads)
   const int r0 = rowp[i], r1 = rowp[i + 1], di = diag[i];
   for (int t = tid; t < r1 - r0; t += blockDim.x) scol[t] = cols[r0 + t];
   __syncthreads();
-  int kn = r0 < di ? scol[0] : 0;  // next k and its U-part bounds
-  int dn = r0 < di ? diag[kn] : 0, un = r0 < di ? rowp[kn + 1] : 0;
-  for (int kk = r0; kk < di; kk++) {
-    const int dk = dn, u0 = dn + 1, u1 = un;
-    if (kk + 1 < di) {  // bounds of the next k, loaded ahead
-      kn = scol[kk + 1 - r0];
-      dn = diag[kn];
-      un = rowp[kn + 1];
+  // everything about pivot row k (its U part, U_kk, the positions of its
+  // columns in row i) is final before this row starts, so it is staged for
+  // the next k while the current k's update runs
+  auto stage = [&](int kk, int buf) {
+    const int k = scol[kk - r0];
+    const int u0 = diag[k] + 1, u1 = rowp[k + 1];
+    if (tid == 0) {
+      ub[buf][0] = u0;
+      ub[buf][1] = u1;
     }
+    if (tid < 36) udg[buf][tid] = vals[36 * (size_t)diag[k] + tid];
+    for (int t = tid; t < u1 - u0; t += blockDim.x) {
+      const int j = cols[u0 + t];
+      int lo = kk + 1 - r0, hi = r1 - r0;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (scol[mid] < j)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      pos[buf][t] = (lo < r1 - r0 && scol[lo] == j) ? r0 + lo : -1;
+    }
+  };
+  if (r0 < di) stage(r0, 0);
+  __syncthreads();
+  for (int kk = r0; kk < di; kk++) {
+    const int cur = (kk - r0) & 1;
     T* Aik = vals + 36 * (size_t)kk;
-    const T* Ukk = vals + 36 * (size_t)dk;
-    // L_ik = A_ik U_kk^-1 (row r by forward substitution over columns)
+    const T* Ukk = udg[cur];
+    // L_ik = A_ik U_kk^-1 (row r by forward substitution over columns) on
+    // lanes 0-5, the next k staged by the other lanes meanwhile
     if (tid < 6) {
       const int r = tid;
       T x[6];
@@ -103,26 +125,35 @@ __global__ void __launch_bounds__(kIluThreads)
         Aik[6 * r + c] = x[c];
         lik[6 * r + c] = x[c];
       }
-    }
-    // positions in row i of the U part of row k
-    const int nu = u1 - u0;
-    for (int t = tid; t < nu; t += blockDim.x) {
-      const int j = cols[u0 + t];
-      int lo = kk + 1 - r0, hi = r1 - r0;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (scol[mid] < j)
-          lo = mid + 1;
-        else
-          hi = mid;
+    } else if (kk + 1 < di) {
+      // lanes 6.. stage the next pivot row (tid - 6 as the local index)
+      const int k = scol[kk + 1 - r0];
+      const int u0 = diag[k] + 1, u1 = rowp[k + 1];
+      const int tl = tid - 6, nth = blockDim.x - 6;
+      if (tl == 0) {
+        ub[cur ^ 1][0] = u0;
+        ub[cur ^ 1][1] = u1;
       }
-      pos[t] = (lo < r1 - r0 && scol[lo] == j) ? r0 + lo : -1;
+      if (tl < 36) udg[cur ^ 1][tl] = vals[36 * (size_t)diag[k] + tl];
+      for (int t = tl; t < u1 - u0; t += nth) {
+        const int j = cols[u0 + t];
+        int lo = kk + 2 - r0, hi = r1 - r0;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (scol[mid] < j)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        pos[cur ^ 1][t] = (lo < r1 - r0 && scol[lo] == j) ? r0 + lo : -1;
+      }
     }
     __syncthreads();
     // A_ij -= L_ik U_kj
+    const int u0 = ub[cur][0], nu = ub[cur][1] - u0;
     for (int w = tid; w < 36 * nu; w += blockDim.x) {
       const int t = w / 36, e = w % 36, r = e / 6, c = e % 6;
-      const int p = pos[t];
+      const int p = pos[cur][t];
       if (p < 0) continue;
       const T* Ukj = vals + 36 * (size_t)(u0 + t);
       T v = 0;
