@@ -65,7 +65,7 @@ def parse():
     # served request mix (separate JSON line; see serve_main)
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--requests", type=int, default=32)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--rate", type=float, default=0.0,
                     help="Poisson arrival rate (requests/s); 0: all at t=0")
@@ -128,6 +128,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.serve:
+        # one hardware queue per worker stream (HIP's default is 4), set
+        # before the HIP runtime starts
+        q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        if q < min(args.slots, 16):
+            os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.slots, 16))
         serve_main(args)
         return
     if args.profile_kernels:

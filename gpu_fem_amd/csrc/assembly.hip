@@ -674,15 +674,19 @@ int gf_asm_plan_create(gf_context ctx, int nelems, int nnodes,
   }
   for (int k = 0; k < nnzb && !pl->zero_k; k++) pl->zero_k = !kt[k];
   for (int i = 0; i < nnodes && !pl->zero_res; i++) pl->zero_res = !rt[i];
-  if (hipMalloc(&pl->elems, sizeof(int) * nelems) != hipSuccess ||
-      hipMalloc(&pl->mask, sizeof(int) * nelems) != hipSuccess) {
+  int st = gf::dmalloc(ctx, (void**)&pl->elems, sizeof(int) * nelems);
+  if (!st) st = gf::dmalloc(ctx, (void**)&pl->mask, sizeof(int) * nelems);
+  if (st) {
     gf_asm_plan_destroy(pl);
-    return gf::set_error(ctx, GF_ERR_NOMEM, "asm_plan_create: hipMalloc");
+    return st;
   }
-  if (hipMemcpy(pl->elems, elems.data(), sizeof(int) * nelems,
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(pl->mask, mask.data(), sizeof(int) * nelems,
-                hipMemcpyHostToDevice) != hipSuccess) {
+  // copies on the context's stream (a plain hipMemcpy would also wait for
+  // the other streams of the device); the host vectors live until the sync
+  if (hipMemcpyAsync(pl->elems, elems.data(), sizeof(int) * nelems,
+                     hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(pl->mask, mask.data(), sizeof(int) * nelems,
+                     hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
     gf_asm_plan_destroy(pl);
     return gf::set_error(ctx, GF_ERR_DEVICE, "asm_plan_create: upload");
   }
@@ -692,8 +696,8 @@ int gf_asm_plan_create(gf_context ctx, int nelems, int nnodes,
 
 int gf_asm_plan_destroy(gf_asm_plan pl) {
   if (!pl) return GF_OK;
-  hipFree(pl->elems);
-  hipFree(pl->mask);
+  gf::dfree(pl->ctx, pl->elems);
+  gf::dfree(pl->ctx, pl->mask);
   delete pl;
   return GF_OK;
 }

@@ -322,7 +322,7 @@ int gf_bsr_resid_f64(gf_context ctx, int nnodes, const int* rowp,
   const int nblk = (int)gf::div_up(nnodes, 40);
   char* buf = nullptr;
   const size_t part = sizeof(double) * (size_t)nblk;
-  GF_HIP(ctx, hipMalloc(&buf, 3 * part + 2 * sizeof(double)));
+  GF_TRY(gf::dmalloc(ctx, (void**)&buf, 3 * part + 2 * sizeof(double)));
   double* pmax = (double*)buf;
   long long* pidx = (long long*)(buf + part);
   double* pss = (double*)(buf + 2 * part);
@@ -340,7 +340,7 @@ int gf_bsr_resid_f64(gf_context ctx, int nnodes, const int* rowp,
                        "resid copy");
   if (st == GF_OK)
     st = gf::check_hip(ctx, hipStreamSynchronize(ctx->stream), "resid sync");
-  hipFree(buf);
+  gf::dfree(ctx, buf);
   if (st != GF_OK) return st;
   if (max_entry) *max_entry = h[0];
   if (norm2) *norm2 = h[1];

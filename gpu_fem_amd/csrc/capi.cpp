@@ -17,19 +17,21 @@
 #include "krylov.h"
 #include "symbolic.h"
 
-extern "C" {
-
-int gf_version(void) { return 1; }
-
-int gf_context_create(int device, gf_context* out) {
+namespace gf {
+// worker = true: a serving worker's context, whose stream does not
+// synchronise with the legacy default stream and whose freed device blocks
+// are cached (see gf_context_s)
+int context_create(int device, bool worker, gf_context* out) {
   if (!out) return GF_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
     return GF_ERR_DEVICE;
   auto* c = new gf_context_s;
   c->device = device;
+  c->caching = worker;
   if (hipSetDevice(device) != hipSuccess ||
-      hipStreamCreate(&c->stream) != hipSuccess) {
+      (worker ? hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)
+              : hipStreamCreate(&c->stream)) != hipSuccess) {
     delete c;
     return GF_ERR_DEVICE;
   }
@@ -39,9 +41,21 @@ int gf_context_create(int device, gf_context* out) {
   *out = c;
   return GF_OK;
 }
+}  // namespace gf
+
+extern "C" {
+
+int gf_version(void) { return 1; }
+
+int gf_context_create(int device, gf_context* out) {
+  return gf::context_create(device, false, out);
+}
 
 int gf_context_destroy(gf_context c) {
   if (!c) return GF_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  gf::release_cache(c);
+  if (c->pinned) hipHostFree(c->pinned);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return GF_OK;
@@ -97,7 +111,7 @@ namespace {
 
 template <typename T>
 int upload(gf_context ctx, T** dst, const T* src, size_t n) {
-  GF_HIP(ctx, hipMalloc((void**)dst, sizeof(T) * (n ? n : 1)));
+  GF_TRY(gf::dmalloc(ctx, (void**)dst, sizeof(T) * (n ? n : 1)));
   if (n && src)
     GF_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice,
                                ctx->stream));
@@ -106,8 +120,7 @@ int upload(gf_context ctx, T** dst, const T* src, size_t n) {
 
 template <typename T>
 int dalloc(gf_context ctx, T** dst, size_t n) {
-  GF_HIP(ctx, hipMalloc((void**)dst, sizeof(T) * (n ? n : 1)));
-  return GF_OK;
+  return gf::dmalloc(ctx, (void**)dst, sizeof(T) * (n ? n : 1));
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -197,8 +210,7 @@ int gf_problem_destroy(gf_problem p) {
                   p->tcols,  p->tmap,  p->kvals,   p->mvals,   p->kvals32,
                   p->mvals32, p->rhs,  p->sol,     p->loads,   p->work64,
                   p->work64b, p->t32a, p->t32b,    p->t32c};
-  for (void* b : bufs)
-    if (b) hipFree(b);
+  for (void* b : bufs) gf::dfree(p->ctx, b);
   if (p->ilu) gf_ilu_destroy(p->ilu);
   if (p->plan) gf_asm_plan_destroy(p->plan);
   p->kw.release();
@@ -381,12 +393,12 @@ extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
   double *w64 = nullptr, *w64b = nullptr;
   float *a = nullptr, *bb = nullptr, *c = nullptr;
   int st = kw.init(ctx, N, restart, mixed);
-  if (!st) st = gf::check_hip(ctx, hipMalloc(&w64, sizeof(double) * N), "malloc");
-  if (!st) st = gf::check_hip(ctx, hipMalloc(&w64b, sizeof(double) * N), "malloc");
+  if (!st) st = gf::dmalloc(ctx, (void**)&w64, sizeof(double) * N);
+  if (!st) st = gf::dmalloc(ctx, (void**)&w64b, sizeof(double) * N);
   if (!st && mixed) {
-    st = gf::check_hip(ctx, hipMalloc(&a, sizeof(float) * N), "malloc");
-    if (!st) st = gf::check_hip(ctx, hipMalloc(&bb, sizeof(float) * N), "malloc");
-    if (!st) st = gf::check_hip(ctx, hipMalloc(&c, sizeof(float) * N), "malloc");
+    st = gf::dmalloc(ctx, (void**)&a, sizeof(float) * N);
+    if (!st) st = gf::dmalloc(ctx, (void**)&bb, sizeof(float) * N);
+    if (!st) st = gf::dmalloc(ctx, (void**)&c, sizeof(float) * N);
   }
   gf::GmresResult gr;
   if (!st) {
@@ -408,11 +420,11 @@ extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
     st = gf::gmres_solve(kw, op, b, x, max_iters, rtol, precision, &gr);
   }
   hipStreamSynchronize(ctx->stream);
-  hipFree(w64);
-  hipFree(w64b);
-  hipFree(a);
-  hipFree(bb);
-  hipFree(c);
+  gf::dfree(ctx, w64);
+  gf::dfree(ctx, w64b);
+  gf::dfree(ctx, a);
+  gf::dfree(ctx, bb);
+  gf::dfree(ctx, c);
   kw.release();
   if (iterations) *iterations = gr.iterations;
   if (rel_resid) *rel_resid = gr.rel_resid;

@@ -149,17 +149,37 @@ __global__ void __launch_bounds__(kIluThreads)
       }
     }
     __syncthreads();
-    // A_ij -= L_ik U_kj
+    // A_ij -= L_ik U_kj, kBatch entries per thread at a time: every load of
+    // a batch is issued before its stores (the U_kj blocks and the targets
+    // are in the same array, so the compiler would otherwise serialise each
+    // entry's loads behind the previous entry's store)
     const int u0 = ub[cur][0], nu = ub[cur][1] - u0;
-    for (int w = tid; w < 36 * nu; w += blockDim.x) {
-      const int t = w / 36, e = w % 36, r = e / 6, c = e % 6;
-      const int p = pos[cur][t];
-      if (p < 0) continue;
-      const T* Ukj = vals + 36 * (size_t)(u0 + t);
-      T v = 0;
+    constexpr int kBatch = 4;
+    for (int w0 = 0; w0 < 36 * nu; w0 += kBatch * (int)blockDim.x) {
+      T uk[kBatch][6], av[kBatch];
+      long long dst[kBatch];
+      int ee[kBatch];
 #pragma unroll
-      for (int m = 0; m < 6; m++) v += lik[6 * r + m] * Ukj[6 * m + c];
-      vals[36 * (size_t)p + e] -= v;
+      for (int bq = 0; bq < kBatch; bq++) {
+        const int w = w0 + bq * blockDim.x + tid;
+        const int t = w / 36, e = w % 36, c = e % 6;
+        const int p = w < 36 * nu ? pos[cur][t] : -1;
+        dst[bq] = p < 0 ? -1 : 36LL * p + e;
+        ee[bq] = e;
+        const T* Ukj = vals + 36 * (size_t)(u0 + (p < 0 ? 0 : t));
+#pragma unroll
+        for (int m = 0; m < 6; m++) uk[bq][m] = p < 0 ? T(0) : Ukj[6 * m + c];
+        av[bq] = p < 0 ? T(0) : vals[(size_t)dst[bq]];
+      }
+#pragma unroll
+      for (int bq = 0; bq < kBatch; bq++) {
+        if (dst[bq] < 0) continue;
+        const int r = ee[bq] / 6;
+        T v = 0;
+#pragma unroll
+        for (int m = 0; m < 6; m++) v += lik[6 * r + m] * uk[bq][m];
+        vals[(size_t)dst[bq]] = av[bq] - v;
+      }
     }
     __syncthreads();
   }
@@ -252,6 +272,9 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 #endif
 #ifndef GPUFEM_SWEEP_MINWAVES
 #define GPUFEM_SWEEP_MINWAVES 1
+#endif
+#ifndef GPUFEM_SWEEP_CRIT
+#define GPUFEM_SWEEP_CRIT 1  // wait on the latest-level dependency first
 #endif
 #ifndef GPUFEM_SWEEP_BACKOFF
 #define GPUFEM_SWEEP_BACKOFF 8  // longest sleep between polls, x 64 cycles
@@ -366,7 +389,9 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
       const bool live = pb + grp < p1;
       const int4 m = mnext;
       if (pb + kRows + grp < p1) mnext = meta[pb + kRows + grp];
-      const int i = m.x, k0 = live ? m.y : 0, k1 = live ? m.z : 0, di = m.w;
+      const int i = m.x, k0 = live ? m.y : 0, k1 = live ? m.z : 0;
+      const int di = kLower ? m.z : m.y - 1;
+      const int crit = live ? m.w : -1;
       T s = (live && gl < 6) ? b[6 * (size_t)i + gl] : T(0);
       if (kLower && fill_next && live && gl < 6)
         reinterpret_cast<B*>(fill_next)[6 * (size_t)i + gl] =
@@ -380,6 +405,28 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
       // are read (lane q: blocks q, q+G, ...) and the dependencies' entries
       // are loaded at once, polled until no sentinel is left, and parked in
       // LDS; then lanes (t, r) accumulate A[q][r][:] . x[col_q][:].
+#if GPUFEM_SWEEP_CRIT
+      // wait for the critical dependency alone (one load per row group per
+      // poll) before the row's dependencies are gathered: a wavefront that
+      // is levels ahead then adds little polling traffic
+      {
+        const int cj = (crit >= 0 && !cached(crit)) ? crit : -1;
+        B v = cj >= 0 ? load_bits(x + 6 * (size_t)cj) : B(0);
+        unsigned spins = 0;
+        int nap = 1;
+        while (__any(v == Sentinel<T>::value)) {
+          if (++spins > kSpinLimit) {
+            if (lane == 0)
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(1);
+          nap = nap * 2 > GPUFEM_SWEEP_BACKOFF ? GPUFEM_SWEEP_BACKOFF : nap * 2;
+          if (v == Sentinel<T>::value) v = load_bits(x + 6 * (size_t)cj);
+        }
+      }
+#endif
       T acc = T(0);
       const int nb = k1 - k0;
       int npass = (nb + kGather - 1) / kGather;
@@ -590,15 +637,44 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
   h->nlchunk = (int)lch.size() - 1;
   h->nuchunk = (int)uch.size() - 1;
   // row descriptors in hand-out order: one 16-byte load per row
+  // {row, first block, end block, critical dependency}: the diagonal block
+  // is the end (L) or one before the first (U); the critical dependency is
+  // the row's dependency of the highest level, latest in hand-out order
+  // (-1 for none), which is usually the last one to be solved
   std::vector<int4> lm(n), um(n);
-  for (int p = 0; p < n; p++) {
-    const int li = use_chains ? p : lrows[p];
-    lm[p] = make_int4(li, rowp[li], diag[li], diag[li]);
-    const int ui = use_chains ? n - 1 - p : urows[p];
-    um[p] = make_int4(ui, diag[ui] + 1, rowp[ui + 1], diag[ui]);
+  {
+    std::vector<int> lev(n), ord(n);
+    auto crit_of = [&](const std::vector<int>& lp, const std::vector<int>& rows,
+                       bool lower, std::vector<int>& out) {
+      for (size_t l = 0; l + 1 < lp.size(); l++)
+        for (int q = lp[l]; q < lp[l + 1]; q++) lev[rows[q]] = (int)l;
+      for (int q = 0; q < n; q++) ord[use_chains ? (lower ? q : n - 1 - q) : rows[q]] = q;
+      out.assign(n, -1);
+      for (int i = 0; i < n; i++) {
+        const int k0 = lower ? rowp[i] : diag[i] + 1;
+        const int k1 = lower ? diag[i] : rowp[i + 1];
+        int best = -1;
+        for (int k = k0; k < k1; k++) {
+          const int j = cols[k];
+          if (best < 0 || lev[j] > lev[best] ||
+              (lev[j] == lev[best] && ord[j] > ord[best]))
+            best = j;
+        }
+        out[i] = best;
+      }
+    };
+    std::vector<int> lcrit, ucrit;
+    crit_of(h->lptr, lrows, true, lcrit);
+    crit_of(h->uptr, urows, false, ucrit);
+    for (int p = 0; p < n; p++) {
+      const int li = use_chains ? p : lrows[p];
+      lm[p] = make_int4(li, rowp[li], diag[li], lcrit[li]);
+      const int ui = use_chains ? n - 1 - p : urows[p];
+      um[p] = make_int4(ui, diag[ui] + 1, rowp[ui + 1], ucrit[ui]);
+    }
   }
   auto up = [&](int** dst, const int* src, size_t cnt) -> int {
-    GF_HIP(ctx, hipMalloc(dst, sizeof(int) * (cnt ? cnt : 1)));
+    GF_TRY(gf::dmalloc(ctx, (void**)dst, sizeof(int) * (cnt ? cnt : 1)));
     GF_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(int) * cnt,
                                hipMemcpyHostToDevice, ctx->stream));
     return GF_OK;
@@ -612,10 +688,9 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
       (st = up((int**)&h->lmeta, (const int*)lm.data(), 4 * (size_t)n)) ||
       (st = up((int**)&h->umeta, (const int*)um.data(), 4 * (size_t)n)) ||
       (st = up(&h->uchunk, uch.data(), uch.size())) ||
-      (st = gf::check_hip(ctx, hipMalloc(&h->pivot, sizeof(int)), "malloc")) ||
-      (st = gf::check_hip(ctx, hipMalloc(&h->counter, sizeof(unsigned long long)),
-                          "malloc")) ||
-      (st = gf::check_hip(ctx, hipMalloc(&h->err, sizeof(int)), "malloc")) ||
+      (st = gf::dmalloc(ctx, (void**)&h->pivot, sizeof(int))) ||
+      (st = gf::dmalloc(ctx, (void**)&h->counter, sizeof(unsigned long long))) ||
+      (st = gf::dmalloc(ctx, (void**)&h->err, sizeof(int))) ||
       (st = gf::check_hip(ctx, hipMemsetAsync(h->counter, 0, sizeof(unsigned long long),
                                               ctx->stream), "memset")) ||
       (st = gf::check_hip(ctx, hipMemsetAsync(h->err, 0, sizeof(int), ctx->stream),
@@ -624,9 +699,15 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
     return st;
   }
   {
-    // enough wavefronts to cover the device, capped by the chunks
-    const int waves = std::max(1, std::min(std::max(h->nlchunk, h->nuchunk),
-                                           target_waves));
+    // enough wavefronts to cover the device, capped by the chunks and by
+    // eight mean level widths: wavefronts further ahead than that mostly
+    // wait, and on small matrices they would take the device from
+    // concurrent solves on other streams
+    const int nlev = (int)std::max(h->lptr.size(), h->uptr.size()) - 1;
+    const int mean_w = (n + std::max(nlev, 1) - 1) / std::max(nlev, 1);
+    const int waves = std::max(
+        1, std::min({std::max(h->nlchunk, h->nuchunk), target_waves,
+                     std::max(64, 8 * mean_w)}));
     h->grid = (waves + 3) / 4;
     const char* env = getenv("GPUFEM_TRSV");
     h->level_sweeps = env && std::string(env) == "level";
@@ -642,18 +723,18 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
 
 int gf_ilu_destroy(gf_ilu h) {
   if (!h) return GF_OK;
-  hipFree(h->rowp);
-  hipFree(h->cols);
-  hipFree(h->diag);
-  hipFree(h->lrows);
-  hipFree(h->urows);
-  hipFree(h->pivot);
-  hipFree(h->lchunk);
-  hipFree(h->lmeta);
-  hipFree(h->umeta);
-  hipFree(h->uchunk);
-  hipFree(h->counter);
-  hipFree(h->err);
+  gf::dfree(h->ctx, h->rowp);
+  gf::dfree(h->ctx, h->cols);
+  gf::dfree(h->ctx, h->diag);
+  gf::dfree(h->ctx, h->lrows);
+  gf::dfree(h->ctx, h->urows);
+  gf::dfree(h->ctx, h->pivot);
+  gf::dfree(h->ctx, h->lchunk);
+  gf::dfree(h->ctx, h->lmeta);
+  gf::dfree(h->ctx, h->umeta);
+  gf::dfree(h->ctx, h->uchunk);
+  gf::dfree(h->ctx, h->counter);
+  gf::dfree(h->ctx, h->err);
   delete h;
   return GF_OK;
 }
@@ -681,7 +762,9 @@ int gf_ilu_check(gf_ilu h) {
   GF_HIP(h->ctx, hipStreamSynchronize(h->ctx->stream));
   if (e) {
     const int zero = 0;
-    hipMemcpy(h->err, &zero, sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpyAsync(h->err, &zero, sizeof(int), hipMemcpyHostToDevice,
+                   h->ctx->stream);
+    hipStreamSynchronize(h->ctx->stream);
     return gf::set_error(h->ctx, GF_ERR_TIMEOUT,
                          "triangular sweep: a dependency wait expired");
   }

@@ -243,7 +243,7 @@ int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
   N = N_;
   m = m_;
   auto dm = [&](void** p, size_t bytes) {
-    return check_hip(ctx, hipMalloc(p, bytes ? bytes : 8), "krylov malloc");
+    return dmalloc(ctx, p, bytes ? bytes : 8);
   };
   GF_TRY(dm((void**)&V, sizeof(double) * N * (m + 1)));
   GF_TRY(dm((void**)&w, sizeof(double) * N));
@@ -263,8 +263,10 @@ int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
     GF_TRY(dm((void**)&b32, sizeof(float) * N));
     GF_TRY(dm((void**)&c32, sizeof(float) * N));
   }
-  GF_TRY(check_hip(ctx, hipHostMalloc((void**)&host_scal, sizeof(double) * 4),
-                   "hipHostMalloc"));
+  if (!ctx->pinned)
+    GF_TRY(check_hip(ctx, hipHostMalloc((void**)&ctx->pinned, sizeof(double) * 8),
+                     "hipHostMalloc"));
+  host_scal = ctx->pinned;
   GF_TRY(check_hip(ctx, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming),
                    "event"));
   GF_TRY(check_hip(ctx, hipEventCreateWithFlags(&ev[1], hipEventDisableTiming),
@@ -273,23 +275,22 @@ int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
 }
 
 void KrylovWork::release() {
-  hipFree(V);
-  hipFree(w);
-  hipFree(z);
-  hipFree(r);
-  hipFree(tmp);
-  hipFree(h);
-  hipFree(h2);
-  hipFree(nrm);
-  hipFree(y);
-  hipFree(state);
-  hipFree(partial);
-  hipFree(scal);
-  hipFree(gate);
-  hipFree(a32);
-  hipFree(b32);
-  hipFree(c32);
-  if (host_scal) hipHostFree(host_scal);
+  dfree(ctx, V);
+  dfree(ctx, w);
+  dfree(ctx, z);
+  dfree(ctx, r);
+  dfree(ctx, tmp);
+  dfree(ctx, h);
+  dfree(ctx, h2);
+  dfree(ctx, nrm);
+  dfree(ctx, y);
+  dfree(ctx, state);
+  dfree(ctx, partial);
+  dfree(ctx, scal);
+  dfree(ctx, gate);
+  dfree(ctx, a32);
+  dfree(ctx, b32);
+  dfree(ctx, c32);
   if (ev[0]) hipEventDestroy(ev[0]);
   if (ev[1]) hipEventDestroy(ev[1]);
   *this = KrylovWork();

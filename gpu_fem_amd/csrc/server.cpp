@@ -77,7 +77,7 @@ size_t estimate_bytes(const Request& r) {
 
 void worker_main(gf_server_s* s) {
   gf_context ctx = nullptr;
-  if (gf_context_create(s->device, &ctx) != GF_OK) return;
+  if (gf::context_create(s->device, true, &ctx) != GF_OK) return;
   for (;;) {
     Request req;
     {
