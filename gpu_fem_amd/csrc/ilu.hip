@@ -23,6 +23,7 @@
 //     deadlock.  The level-by-level sweep stays available (GPUFEM_TRSV=level)
 //     as an independent schedule for parity checks.
 #include <algorithm>
+#include <mutex>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -38,9 +39,10 @@ struct gf_ilu_s {
   std::vector<int> lptr, uptr;             // level offsets (host)
   int* pivot = nullptr;                    // first zero-pivot row (device)
   int max_row_len = 0;
+  std::vector<int> lmaxlen;                // longest row of each L level
   // sync-free sweep state
   int *lchunk = nullptr, *uchunk = nullptr; // chunk offsets (see below)
-  int4 *lmeta = nullptr, *umeta = nullptr;  // {row, k0, k1, diag} per
+  int4 *lmeta = nullptr, *umeta = nullptr;  // {row, k0, k1, crit} per
                                             // position in hand-out order
   int nlchunk = 0, nuchunk = 0;
   bool lchains = false, uchains = false;  // chunks are chains in row order
@@ -218,6 +220,165 @@ __global__ void __launch_bounds__(kIluThreads)
   }
 }
 
+// The same factorisation with row i held in LDS for the whole row: L_ik is
+// formed from, and the updates A_ij -= L_ik U_kj are applied to, the LDS
+// copy, so the only global accesses inside the pivot loop are the final rows
+// k (U_kk, the U part and its column indices), and each thread's U_kj values
+// for pivot k are loaded before L_ik is formed.  The row is written back
+// once at the end.  Arithmetic is that of ilu_row_kernel, operation for
+// operation.  Dynamic LDS: 36 * nb values then nb column indices.
+constexpr int kIluPairs = 4;  // (block, column) pairs of U_kj in registers
+template <typename T>
+__global__ void __launch_bounds__(kIluThreads)
+    ilu_row_lds_kernel(const int* __restrict__ level_rows, int nrows,
+                       const int* __restrict__ rowp,
+                       const int* __restrict__ cols,
+                       const int* __restrict__ diag, T* vals, int* pivot) {
+  extern __shared__ __align__(16) unsigned char ilu_dyn[];
+  __shared__ int pos[2][kMaxRowBlocks];  // local positions in row i, or -1
+  __shared__ int ub[2][2];
+  __shared__ T udg[2][36];
+  __shared__ T lik[36];
+  const int li = blockIdx.x;
+  if (li >= nrows) return;
+  const int i = level_rows[li];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int r0 = rowp[i], r1 = rowp[i + 1], nb = r1 - r0, dl = diag[i] - r0;
+  T* row = reinterpret_cast<T*>(ilu_dyn);
+  int* scol = reinterpret_cast<int*>(row + 36 * (size_t)nb);
+  for (int e = tid; e < 36 * nb; e += nth) row[e] = vals[36 * (size_t)r0 + e];
+  for (int t = tid; t < nb; t += nth) scol[t] = cols[r0 + t];
+  __syncthreads();
+  // pattern of pivot row scol[kl] into buffer buf, by threads t0.. of nt
+  auto stage = [&](int kl, int buf, int t0, int nt) {
+    const int k = scol[kl];
+    const int u0 = diag[k] + 1, u1 = rowp[k + 1];
+    if (t0 == 0) {
+      ub[buf][0] = u0;
+      ub[buf][1] = u1;
+    }
+    if (t0 < 36) udg[buf][t0] = vals[36 * (size_t)diag[k] + t0];
+    for (int t = t0; t < u1 - u0; t += nt) {
+      const int j = cols[u0 + t];
+      int lo = kl + 1, hi = nb;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (scol[mid] < j)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      pos[buf][t] = (lo < nb && scol[lo] == j) ? lo : -1;
+    }
+  };
+  if (dl > 0) stage(0, 0, tid, nth);
+  __syncthreads();
+  for (int kl = 0; kl < dl; kl++) {
+    const int cur = kl & 1;
+    const int u0 = ub[cur][0], nu = ub[cur][1] - u0;
+    // this thread's U_kj columns (pairs w = tid + s * nth: block w / 6,
+    // column w % 6), loaded while L_ik is formed
+    T uv[kIluPairs][6];
+#pragma unroll
+    for (int sp = 0; sp < kIluPairs; sp++) {
+      const int w = tid + sp * nth;
+      const bool ok = w < 6 * nu;
+      const T* U = vals + 36 * (size_t)(u0 + (ok ? w / 6 : 0)) + (ok ? w % 6 : 0);
+#pragma unroll
+      for (int m = 0; m < 6; m++) uv[sp][m] = ok ? U[6 * m] : T(0);
+    }
+    if (tid < 6) {
+      const int r = tid;
+      const T* Ukk = udg[cur];
+      T* Aik = row + 36 * kl;
+      T x[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        T v = Aik[6 * r + c];
+#pragma unroll
+        for (int m = 0; m < c; m++) v -= x[m] * Ukk[6 * m + c];
+        x[c] = v / Ukk[6 * c + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        Aik[6 * r + c] = x[c];
+        lik[6 * r + c] = x[c];
+      }
+    } else if (kl + 1 < dl) {
+      stage(kl + 1, cur ^ 1, tid - 6, nth - 6);
+    }
+    __syncthreads();
+    // A_ij -= L_ik U_kj on the LDS row
+#pragma unroll
+    for (int sp = 0; sp < kIluPairs; sp++) {
+      const int w = tid + sp * nth;
+      if (w < 6 * nu) {
+        const int pl = pos[cur][w / 6], c = w % 6;
+        if (pl >= 0) {
+#pragma unroll
+          for (int r = 0; r < 6; r++) {
+            T v = 0;
+#pragma unroll
+            for (int m = 0; m < 6; m++) v += lik[6 * r + m] * uv[sp][m];
+            row[36 * pl + 6 * r + c] -= v;
+          }
+        }
+      }
+    }
+    for (int w = tid + kIluPairs * nth; w < 6 * nu; w += nth) {
+      const int pl = pos[cur][w / 6], c = w % 6;
+      if (pl < 0) continue;
+      const T* U = vals + 36 * (size_t)(u0 + w / 6) + c;
+      T um[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) um[m] = U[6 * m];
+#pragma unroll
+      for (int r = 0; r < 6; r++) {
+        T v = 0;
+#pragma unroll
+        for (int m = 0; m < 6; m++) v += lik[6 * r + m] * um[m];
+        row[36 * pl + 6 * r + c] -= v;
+      }
+    }
+    __syncthreads();
+  }
+  // factor the diagonal block (no pivoting)
+  T* D = row + 36 * dl;
+  if (tid == 0) {
+    for (int c = 0; c < 6; c++) {
+      const T piv = D[6 * c + c];
+      if (piv == T(0)) {
+        atomicMin(pivot, i);
+        break;
+      }
+      for (int r = c + 1; r < 6; r++) {
+        const T l = D[6 * r + c] / piv;
+        D[6 * r + c] = l;
+        for (int m = c + 1; m < 6; m++) D[6 * r + m] -= l * D[6 * c + m];
+      }
+    }
+  }
+  __syncthreads();
+  // U_ij = L_ii^-1 A_ij for the blocks right of the diagonal
+  const int nr = nb - (dl + 1);
+  for (int w = tid; w < 6 * nr; w += nth) {
+    const int t = w / 6, c = w % 6;
+    T* A = row + 36 * (dl + 1 + t);
+    T x[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      T sv = A[6 * r + c];
+#pragma unroll
+      for (int m = 0; m < r; m++) sv -= D[6 * r + m] * x[m];
+      x[r] = sv;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; r++) A[6 * r + c] = x[r];
+  }
+  __syncthreads();
+  for (int e = tid; e < 36 * nb; e += nth) vals[36 * (size_t)r0 + e] = row[e];
+}
+
 // Six lanes per row (lane = scalar row r), ten rows per wave.
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(256)
@@ -273,8 +434,13 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 #ifndef GPUFEM_SWEEP_MINWAVES
 #define GPUFEM_SWEEP_MINWAVES 1
 #endif
+// Waiting on the latest-level dependency alone before gathering the others
+// (GPUFEM_SWEEP_CRIT=1) cuts polling traffic but adds a round trip per row:
+// measured 4.70 vs 3.54 ms a sweep pair on the 400x400 plate (AMD, ILU(12))
+// and 5.53 vs 3.47 ms on the 87x87 plate, and more wavefronts per CU did not
+// recover it (5.07-5.41 ms at 6-16), so it is off.
 #ifndef GPUFEM_SWEEP_CRIT
-#define GPUFEM_SWEEP_CRIT 1  // wait on the latest-level dependency first
+#define GPUFEM_SWEEP_CRIT 0
 #endif
 #ifndef GPUFEM_SWEEP_BACKOFF
 #define GPUFEM_SWEEP_BACKOFF 8  // longest sleep between polls, x 64 cycles
@@ -585,6 +751,11 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
   std::vector<int> lrows, urows;
   gf::level_schedule(n, rowp, cols, true, h->lptr, lrows);
   gf::level_schedule(n, rowp, cols, false, h->uptr, urows);
+  h->lmaxlen.assign(h->lptr.size() - 1, 0);
+  for (size_t l = 0; l + 1 < h->lptr.size(); l++)
+    for (int q = h->lptr[l]; q < h->lptr[l + 1]; q++)
+      h->lmaxlen[l] = std::max(h->lmaxlen[l],
+                               rowp[lrows[q] + 1] - rowp[lrows[q]]);
   // chunks: runs of rows of one level, sized so that a level still spreads
   // over the resident wavefronts (one row per chunk for small levels)
   const int target_waves =
@@ -782,12 +953,44 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   const int big = 0x7fffffff;
   GF_HIP(ctx, hipMemcpyAsync(h->pivot, &big, sizeof(int),
                              hipMemcpyHostToDevice, ctx->stream));
+  // rows whose values fit in LDS (dynamic LDS per level: its longest row)
+  // are factored there; the other levels run ilu_row_kernel
+  static size_t lds_cap[2] = {0, 0};
+  static std::once_flag once[2];
+  const int ti = sizeof(T) == 8 ? 0 : 1;
+  std::call_once(once[ti], [&] {
+    hipFuncAttributes fa{};
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess ||
+        hipFuncGetAttributes(&fa, (const void*)gf::ilu_row_lds_kernel<T>) !=
+            hipSuccess)
+      return;
+    const size_t total = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
+    if (total <= fa.sharedSizeBytes + 1024) return;
+    const size_t cap = total - fa.sharedSizeBytes;
+    if (hipFuncSetAttribute((const void*)gf::ilu_row_lds_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)cap) != hipSuccess) {
+      hipGetLastError();
+      if (cap > 64 * 1024 - fa.sharedSizeBytes) return;
+    }
+    lds_cap[ti] = cap;
+  });
+  const bool no_lds = getenv("GPUFEM_ILU_GLOBAL") != nullptr;
   const int nl = (int)h->lptr.size() - 1;
   for (int l = 0; l < nl; l++) {
     const int cnt = h->lptr[l + 1] - h->lptr[l];
-    hipLaunchKernelGGL(gf::ilu_row_kernel<T>, dim3(cnt), dim3(gf::kIluThreads), 0,
-                       ctx->stream, h->lrows + h->lptr[l], cnt, h->rowp,
-                       h->cols, h->diag, vals, h->pivot);
+    const size_t dyn = (size_t)h->lmaxlen[l] * (36 * sizeof(T) + sizeof(int));
+    if (!no_lds && dyn <= lds_cap[ti])
+      hipLaunchKernelGGL(gf::ilu_row_lds_kernel<T>, dim3(cnt),
+                         dim3(gf::kIluThreads), dyn, ctx->stream,
+                         h->lrows + h->lptr[l], cnt, h->rowp, h->cols, h->diag,
+                         vals, h->pivot);
+    else
+      hipLaunchKernelGGL(gf::ilu_row_kernel<T>, dim3(cnt),
+                         dim3(gf::kIluThreads), 0, ctx->stream,
+                         h->lrows + h->lptr[l], cnt, h->rowp, h->cols, h->diag,
+                         vals, h->pivot);
   }
   GF_TRY(gf::check_launch(ctx, "ilu_row_kernel"));
   int piv = big;

@@ -241,6 +241,29 @@ def test_ilu_factor_and_apply(gpu_ctx, k, reorder):
     ilu.close()
 
 
+@pytest.mark.parametrize("reorder,k", [("rcm", 3), ("amd", 12)])
+def test_ilu_lds_rows_match_global_rows(gpu_ctx, reorder, k, monkeypatch):
+    """Rows factored in LDS (ilu_row_lds_kernel) and in global memory
+    (ilu_row_kernel, GPUFEM_ILU_GLOBAL) perform the same operations: the
+    factors agree bit for bit, FP64 and FP32."""
+    say(f"[gpu] ilu lds vs global rows {reorder} ILU({k}) plate 24x24")
+    m = M.plate(24, 24, bc="ss", load="sine")
+    s = oracle_problem(m, reorder, k)
+    ilu = G.Ilu(gpu_ctx, s["mrp"], s["mcp"])
+    for dt in (np.float64, np.float32):
+        d_a = dev(s["mv"].astype(dt))
+        ilu.factor(d_a)
+        gpu_ctx.sync()
+        monkeypatch.setenv("GPUFEM_ILU_GLOBAL", "1")
+        d_b = dev(s["mv"].astype(dt))
+        ilu.factor(d_b)
+        gpu_ctx.sync()
+        monkeypatch.delenv("GPUFEM_ILU_GLOBAL")
+        a, b = host(d_a), host(d_b)
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    ilu.close()
+
+
 def test_zero_pivot_reported(gpu_ctx):
     say("[gpu] zero pivot detection")
     m = M.plate(3, 3)
