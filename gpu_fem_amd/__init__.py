@@ -37,7 +37,8 @@ class GfError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("reorder", ctypes.c_int), ("lev_fill", ctypes.c_int),
                 ("restart", ctypes.c_int), ("max_iters", ctypes.c_int),
-                ("rtol", ctypes.c_double), ("precision", ctypes.c_int)]
+                ("rtol", ctypes.c_double), ("precision", ctypes.c_int),
+                ("stall_cycles", ctypes.c_int)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -419,7 +420,7 @@ class Server:
     slots with their own streams."""
 
     STATS = ["completed", "failed", "queued", "active", "p50_s", "p99_s",
-             "peak_bytes", "elapsed_s"]
+             "peak_bytes", "elapsed_s", "escalated"]
 
     def __init__(self, budget_gb=200.0, slots=4, device=0):
         h = _P()
@@ -458,7 +459,7 @@ class Server:
         return rid.value, x[:n].copy(), info, lat.value
 
     def stats(self):
-        out = np.zeros(8)
+        out = np.zeros(9)
         check(lib().gf_server_stats(self.h, ptr(out)))
         return dict(zip(self.STATS, out.tolist()))
 

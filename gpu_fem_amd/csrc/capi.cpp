@@ -80,6 +80,7 @@ void gf_options_default(gf_options* o) {
   o->max_iters = 5000;
   o->rtol = 1e-8;
   o->precision = 0;
+  o->stall_cycles = 0;
 }
 
 }  // extern "C"
@@ -321,6 +322,7 @@ int gf_problem_solve(gf_problem p, const double* loads, double* x,
   op.t32c = p->t32c;
   gf::GmresResult gr;
   const int st = gf::gmres_solve(p->kw, op, p->rhs, p->sol, p->opt.max_iters,
+                                 p->opt.stall_cycles,
                                  p->opt.rtol, p->opt.precision, &gr);
   // x = inverse permutation of sol: x[old] = sol[perm[old]]
   GF_TRY(gf_permute_vec(ctx, p->nnodes, p->iperm, p->sol, p->work64));
@@ -417,7 +419,7 @@ extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
     op.t32a = a;
     op.t32b = bb;
     op.t32c = c;
-    st = gf::gmres_solve(kw, op, b, x, max_iters, rtol, precision, &gr);
+    st = gf::gmres_solve(kw, op, b, x, max_iters, 0, rtol, precision, &gr);
   }
   hipStreamSynchronize(ctx->stream);
   gf::dfree(ctx, w64);

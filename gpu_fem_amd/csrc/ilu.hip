@@ -222,12 +222,16 @@ __global__ void __launch_bounds__(kIluThreads)
 
 // The same factorisation with row i held in LDS for the whole row: L_ik is
 // formed from, and the updates A_ij -= L_ik U_kj are applied to, the LDS
-// copy, so the only global accesses inside the pivot loop are the final rows
-// k (U_kk, the U part and its column indices), and each thread's U_kj values
-// for pivot k are loaded before L_ik is formed.  The row is written back
-// once at the end.  Arithmetic is that of ilu_row_kernel, operation for
-// operation.  Dynamic LDS: 36 * nb values then nb column indices.
-constexpr int kIluPairs = 4;  // (block, column) pairs of U_kj in registers
+// copy, so the only global accesses inside the pivot loop are reads of the
+// final rows k.  Those are pipelined one pivot ahead: the U-part bounds of
+// every pivot row are read into LDS once; while pivot k is processed, the
+// column indices and U_kk of pivot k+2 and each thread's U_kj columns of
+// pivot k+1 are loaded into registers, and the positions of pivot k+1's
+// columns in row i are searched.  The row is written back once at the end.
+// Arithmetic is that of ilu_row_kernel, operation for operation.
+// Dynamic LDS: 36 * nb values, nb column indices, 2 * nl pivot bounds.
+constexpr int kIluPairs = 4;  // (block, column) pairs of U_kj per thread
+constexpr int kIluCols = 4;   // prefetched column indices per staging thread
 template <typename T>
 __global__ void __launch_bounds__(kIluThreads)
     ilu_row_lds_kernel(const int* __restrict__ level_rows, int nrows,
@@ -236,7 +240,6 @@ __global__ void __launch_bounds__(kIluThreads)
                        const int* __restrict__ diag, T* vals, int* pivot) {
   extern __shared__ __align__(16) unsigned char ilu_dyn[];
   __shared__ int pos[2][kMaxRowBlocks];  // local positions in row i, or -1
-  __shared__ int ub[2][2];
   __shared__ T udg[2][36];
   __shared__ T lik[36];
   const int li = blockIdx.x;
@@ -246,47 +249,87 @@ __global__ void __launch_bounds__(kIluThreads)
   const int r0 = rowp[i], r1 = rowp[i + 1], nb = r1 - r0, dl = diag[i] - r0;
   T* row = reinterpret_cast<T*>(ilu_dyn);
   int* scol = reinterpret_cast<int*>(row + 36 * (size_t)nb);
+  int* ubs = scol + nb;  // [2 * kl]: U part [u0, u1) of pivot row scol[kl]
   for (int e = tid; e < 36 * nb; e += nth) row[e] = vals[36 * (size_t)r0 + e];
-  for (int t = tid; t < nb; t += nth) scol[t] = cols[r0 + t];
+  for (int t = tid; t < nb; t += nth) {
+    const int j = cols[r0 + t];
+    scol[t] = j;
+    if (t < dl) {
+      ubs[2 * t] = diag[j] + 1;
+      ubs[2 * t + 1] = rowp[j + 1];
+    }
+  }
   __syncthreads();
-  // pattern of pivot row scol[kl] into buffer buf, by threads t0.. of nt
-  auto stage = [&](int kl, int buf, int t0, int nt) {
-    const int k = scol[kl];
-    const int u0 = diag[k] + 1, u1 = rowp[k + 1];
-    if (t0 == 0) {
-      ub[buf][0] = u0;
-      ub[buf][1] = u1;
+  const int ts = tid - 6, ns = nth - 6;  // staging threads (tid >= 6)
+  // position of column j among scol[lo..nb), or -1
+  auto find = [&](int j, int lo) {
+    int hi = nb;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (scol[mid] < j)
+        lo = mid + 1;
+      else
+        hi = mid;
     }
-    if (t0 < 36) udg[buf][t0] = vals[36 * (size_t)diag[k] + t0];
-    for (int t = t0; t < u1 - u0; t += nt) {
-      const int j = cols[u0 + t];
-      int lo = kl + 1, hi = nb;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (scol[mid] < j)
-          lo = mid + 1;
-        else
-          hi = mid;
-      }
-      pos[buf][t] = (lo < nb && scol[lo] == j) ? lo : -1;
-    }
+    return (lo < nb && scol[lo] == j) ? lo : -1;
   };
-  if (dl > 0) stage(0, 0, tid, nth);
-  __syncthreads();
-  for (int kl = 0; kl < dl; kl++) {
-    const int cur = kl & 1;
-    const int u0 = ub[cur][0], nu = ub[cur][1] - u0;
-    // this thread's U_kj columns (pairs w = tid + s * nth: block w / 6,
-    // column w % 6), loaded while L_ik is formed
-    T uv[kIluPairs][6];
+  // registers: this thread's U_kj columns for the current and next pivot,
+  // and (staging threads) the column indices and U_kk entry of the pivot
+  // after next
+  T uv[kIluPairs][6], un[kIluPairs][6];
+  int pc[kIluCols];
+  T pd = T(0);
+  auto load_u = [&](int kl, T (&dst)[kIluPairs][6]) {
+    const int u0 = kl < dl ? ubs[2 * kl] : 0;
+    const int nu = kl < dl ? ubs[2 * kl + 1] - u0 : 0;
 #pragma unroll
     for (int sp = 0; sp < kIluPairs; sp++) {
       const int w = tid + sp * nth;
       const bool ok = w < 6 * nu;
-      const T* U = vals + 36 * (size_t)(u0 + (ok ? w / 6 : 0)) + (ok ? w % 6 : 0);
+      const T* U =
+          vals + 36 * (size_t)(u0 + (ok ? w / 6 : 0)) + (ok ? w % 6 : 0);
 #pragma unroll
-      for (int m = 0; m < 6; m++) uv[sp][m] = ok ? U[6 * m] : T(0);
+      for (int m = 0; m < 6; m++) dst[sp][m] = ok ? U[6 * m] : T(0);
     }
+  };
+  auto load_cols = [&](int kl) {
+    const int u0 = kl < dl ? ubs[2 * kl] : 0;
+    const int nu = kl < dl ? ubs[2 * kl + 1] - u0 : 0;
+#pragma unroll
+    for (int q = 0; q < kIluCols; q++) {
+      const int t = ts + q * ns;
+      pc[q] = (ts >= 0 && t < nu) ? cols[u0 + t] : -1;
+    }
+    pd = (ts >= 0 && ts < 36 && kl < dl)
+             ? vals[36 * (size_t)diag[scol[kl]] + ts]
+             : T(0);
+  };
+  // positions and U_kk of pivot kl into buffer buf from the registers
+  auto stage_from_regs = [&](int kl, int buf) {
+    const int u0 = ubs[2 * kl], nu = ubs[2 * kl + 1] - u0;
+#pragma unroll
+    for (int q = 0; q < kIluCols; q++) {
+      const int t = ts + q * ns;
+      if (t < nu) pos[buf][t] = find(pc[q], kl + 1);
+    }
+    for (int t = ts + kIluCols * ns; t < nu; t += ns)
+      pos[buf][t] = find(cols[u0 + t], kl + 1);
+    if (ts < 36) udg[buf][ts] = pd;
+  };
+  if (dl > 0) {
+    // pivot 0 staged directly by all threads
+    const int k = scol[0];
+    const int u0 = ubs[0], nu = ubs[1] - u0;
+    if (tid < 36) udg[0][tid] = vals[36 * (size_t)diag[k] + tid];
+    for (int t = tid; t < nu; t += nth) pos[0][t] = find(cols[u0 + t], 1);
+    load_u(0, uv);
+    load_cols(1);
+  }
+  __syncthreads();
+  for (int kl = 0; kl < dl; kl++) {
+    const int cur = kl & 1;
+    const int u0 = ubs[2 * kl], nu = ubs[2 * kl + 1] - u0;
+    load_u(kl + 1, un);
     if (tid < 6) {
       const int r = tid;
       const T* Ukk = udg[cur];
@@ -305,7 +348,8 @@ __global__ void __launch_bounds__(kIluThreads)
         lik[6 * r + c] = x[c];
       }
     } else if (kl + 1 < dl) {
-      stage(kl + 1, cur ^ 1, tid - 6, nth - 6);
+      stage_from_regs(kl + 1, cur ^ 1);
+      load_cols(kl + 2);
     }
     __syncthreads();
     // A_ij -= L_ik U_kj on the LDS row
@@ -341,6 +385,10 @@ __global__ void __launch_bounds__(kIluThreads)
       }
     }
     __syncthreads();
+#pragma unroll
+    for (int sp = 0; sp < kIluPairs; sp++)
+#pragma unroll
+      for (int m = 0; m < 6; m++) uv[sp][m] = un[sp][m];
   }
   // factor the diagonal block (no pivoting)
   T* D = row + 36 * dl;
@@ -980,7 +1028,8 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   const int nl = (int)h->lptr.size() - 1;
   for (int l = 0; l < nl; l++) {
     const int cnt = h->lptr[l + 1] - h->lptr[l];
-    const size_t dyn = (size_t)h->lmaxlen[l] * (36 * sizeof(T) + sizeof(int));
+    const size_t dyn =
+        (size_t)h->lmaxlen[l] * (36 * sizeof(T) + 3 * sizeof(int));
     if (!no_lds && dyn <= lds_cap[ti])
       hipLaunchKernelGGL(gf::ilu_row_lds_kernel<T>, dim3(cnt),
                          dim3(gf::kIluThreads), dyn, ctx->stream,

@@ -55,7 +55,10 @@ struct GmresResult {
 
 // prec as in Operator: 1 runs GMRES-IR (inner cycles stop at a 1e-5
 // reduction of the FP64 residual), 0 and 2 run plain restarted GMRES.
+// stall_cycles > 0: stop (GF_ERR_NO_CONVERGENCE) once the true residual has
+// fallen by less than 10% over that many restart cycles
 int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
-                int max_iters, double rtol, int prec, GmresResult* res);
+                int max_iters, int stall_cycles, double rtol, int prec,
+                GmresResult* res);
 
 }  // namespace gf

@@ -310,7 +310,8 @@ int KrylovWork::dots(const double* Vb, size_t ldv, int nvec, const double* x,
 }
 
 int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
-                int max_iters, double rtol, int prec, GmresResult* res) {
+                int max_iters, int stall_cycles, double rtol, int prec,
+                GmresResult* res) {
   const bool ir = prec == 1;
   gf_context ctx = W.ctx;
   hipStream_t s = ctx->stream;
@@ -325,6 +326,8 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
   if (bnorm == 0.0) bnorm = 1.0;
   int it = 0, outer = 0, status = GF_ERR_NO_CONVERGENCE;
   double true_rel = 0.0, last_est = 0.0;
+  std::vector<double> hist;  // true relative residual at each cycle start
+  bool stalled = false;
   static const bool verbose = getenv("GPUFEM_GMRES_VERBOSE") != nullptr;
   for (;;) {
     // r = b - A x in FP64 with compensated accumulation
@@ -348,6 +351,14 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
       break;
     }
     if (it >= max_iters) break;
+    // stagnation (when asked for): less than a 10% reduction of the true
+    // residual over the last stall_cycles cycles ends the solve early
+    hist.push_back(true_rel);
+    if (stall_cycles > 0 && (int)hist.size() > stall_cycles &&
+        true_rel > 0.9 * hist[hist.size() - 1 - stall_cycles]) {
+      stalled = true;
+      break;
+    }
     outer++;
     // cycles aim at 0.1 rtol |b|: the true residual of an ill-conditioned
     // preconditioned system trails the Arnoldi estimate, and cycles that stop
@@ -430,8 +441,9 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     res->rel_resid = true_rel;
   }
   if (status == GF_ERR_NO_CONVERGENCE)
-    set_error(ctx, status, "GMRES: %d iterations, rel. residual %.3e > %.1e",
-              it, true_rel, rtol);
+    set_error(ctx, status,
+              "GMRES: %s after %d iterations, rel. residual %.3e > %.1e",
+              stalled ? "stagnated" : "iteration limit", it, true_rel, rtol);
   return status;
 }
 

@@ -55,7 +55,7 @@ struct gf_server_s {
   bool stopping = false;
   int active = 0;
   size_t peak_bytes = 0;
-  long long completed = 0, failed = 0;
+  long long completed = 0, failed = 0, escalated = 0;
   std::vector<double> latencies;
   Clock::time_point t0 = Clock::now();
 };
@@ -74,6 +74,8 @@ size_t estimate_bytes(const Request& r) {
   b += nnzb_k * 12.0 + r.nelems * 96.0 + r.nnodes * 64.0;
   return (size_t)(b * 1.1);
 }
+
+constexpr int kMaxEscalations = 2;
 
 void worker_main(gf_server_s* s) {
   gf_context ctx = nullptr;
@@ -100,18 +102,33 @@ void worker_main(gf_server_s* s) {
     res.id = req.id;
     const auto t_start = Clock::now();
     res.queue_s = std::chrono::duration<double>(t_start - req.arrival).count();
-    gf_problem p = nullptr;
-    int st = gf_problem_create(ctx, req.nnodes, req.nelems, req.conn.data(),
-                               req.xpts.data(), req.data.data(),
-                               (int)req.bcs.size(), req.bcs.data(), &req.opt,
-                               &p);
-    if (st == GF_OK) st = gf_problem_assemble(p);
-    if (st == GF_OK) st = gf_problem_factor(p);
-    if (st == GF_OK) {
-      res.x.resize(6 * (size_t)req.nnodes);
-      st = gf_problem_solve(p, req.loads.data(), res.x.data(), 0, &res.info);
+    // A request whose solve stagnates or whose factorisation breaks down
+    // is run again with twice the ILU level of fill (up to kMaxEscalations
+    // times): on thin shells a higher level converges where ILU(12) does not
+    int st = GF_OK;
+    for (int attempt = 0;; attempt++) {
+      gf_problem p = nullptr;
+      st = gf_problem_create(ctx, req.nnodes, req.nelems, req.conn.data(),
+                             req.xpts.data(), req.data.data(),
+                             (int)req.bcs.size(), req.bcs.data(), &req.opt, &p);
+      if (st == GF_OK) st = gf_problem_assemble(p);
+      if (st == GF_OK) st = gf_problem_factor(p);
+      if (st == GF_OK) {
+        res.x.resize(6 * (size_t)req.nnodes);
+        st = gf_problem_solve(p, req.loads.data(), res.x.data(), 0, &res.info);
+      }
+      if (p) gf_problem_destroy(p);
+      const bool numerical = st == GF_ERR_NO_CONVERGENCE ||
+                             st == GF_ERR_ZERO_PIVOT || st == GF_ERR_BREAKDOWN;
+      if (!numerical || attempt >= kMaxEscalations) break;
+      const size_t before = req.bytes;
+      req.opt.lev_fill = std::min(2 * std::max(req.opt.lev_fill, 1), 64);
+      req.bytes = estimate_bytes(req);
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->in_use += req.bytes - before;
+      s->peak_bytes = std::max(s->peak_bytes, (size_t)s->in_use);
+      s->escalated++;
     }
-    if (p) gf_problem_destroy(p);
     res.status = st;
     res.latency_s =
         std::chrono::duration<double>(Clock::now() - req.arrival).count();
@@ -219,6 +236,7 @@ int gf_server_stats(gf_server s, double* stats) {
   stats[5] = percentile(s->latencies, 0.99);
   stats[6] = (double)s->peak_bytes;
   stats[7] = std::chrono::duration<double>(Clock::now() - s->t0).count();
+  stats[8] = (double)s->escalated;
   return GF_OK;
 }
 

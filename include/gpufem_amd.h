@@ -214,6 +214,10 @@ typedef struct {
   double rtol;       /* ||b - A x|| <= rtol ||b|| in FP64 */
   int precision;     /* 0 FP64; 1 GMRES-IR (FP32 ILU/trsv/SpMV inside,
                         FP64 outer); 2 FP32 ILU/trsv inside FP64 GMRES */
+  int stall_cycles;  /* > 0: stop with GF_ERR_NO_CONVERGENCE once the true
+                        residual fell by less than 10% over this many restart
+                        cycles; 0 (default): run to max_iters as the
+                        reference does */
 } gf_options;
 
 typedef struct {
@@ -262,7 +266,8 @@ int gf_server_submit(gf_server s, long long id, int nnodes, int nelems,
 /* 1 and the result of one finished request, or 0 if none is ready */
 int gf_server_poll(gf_server s, long long* id, double* x, size_t x_len,
                    gf_solve_info* info, double* latency_s);
-/* [completed, failed, queued, active, p50 s, p99 s, peak bytes, elapsed s] */
+/* stats[9] = [completed, failed, queued, active, p50 s, p99 s, peak bytes,
+ * elapsed s, requests re-run with a doubled ILU level of fill] */
 int gf_server_stats(gf_server s, double* stats);
 int gf_server_drain(gf_server s);
 int gf_server_destroy(gf_server s);
