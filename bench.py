@@ -410,10 +410,11 @@ def serve_main(args):
           file=sys.stderr, flush=True)
     # AMD + ILU(12): AMD + ILU(3) stagnates near a relative residual of 1 on
     # the thinner plates and cylinders of this mix
-    # stall_cycles: a solve whose residual falls by less than 10% over 8
-    # restart cycles stops early and is re-run with twice the level of fill
+    # stall_cycles: a solve whose residual falls by less than 10% over 4
+    # restart cycles stops early (and is re-run with twice the level of fill
+    # when it stopped far from the tolerance)
     opts = dict(reorder=2, lev_fill=args.serve_lev_fill, restart=200,
-                rtol=1e-8, max_iters=20000, stall_cycles=8)
+                rtol=1e-8, max_iters=20000, stall_cycles=4)
     ctx = G.Context(0)
     # plain FP64, one at a time
     plain_x, plain_lat, plain_ok = {}, [], 0
@@ -421,8 +422,9 @@ def serve_main(args):
     plain_escalated = 0
     for i, m in enumerate(reqs):
         ts = time.perf_counter()
-        # the server's policy: a solve that stagnates (or an ILU that breaks
-        # down) runs again with twice the level of fill, at most twice
+        # the server's policy: a solve that ends far from the tolerance (or
+        # an ILU or GMRES that breaks down) runs again with twice the level
+        # of fill, at most twice
         o = dict(opts)
         for attempt in range(3):
             p = G.Problem(ctx, m, precision=0, **o)
@@ -434,7 +436,8 @@ def serve_main(args):
                 x, info = None, G.SolveInfo()
                 info.status = e.code
             p.close()
-            if info.status not in (-4, -5, -6) or attempt == 2:
+            far = info.status == -6 and not (info.rel_resid <= 1e3 * o["rtol"])
+            if not (far or info.status in (-4, -5)) or attempt == 2:
                 break
             o["lev_fill"] = min(2 * max(o["lev_fill"], 1), 64)
             plain_escalated += 1

@@ -76,6 +76,7 @@ size_t estimate_bytes(const Request& r) {
 }
 
 constexpr int kMaxEscalations = 2;
+constexpr double kEscalateAbove = 1e3;
 
 void worker_main(gf_server_s* s) {
   gf_context ctx = nullptr;
@@ -102,8 +103,10 @@ void worker_main(gf_server_s* s) {
     res.id = req.id;
     const auto t_start = Clock::now();
     res.queue_s = std::chrono::duration<double>(t_start - req.arrival).count();
-    // A request whose solve stagnates or whose factorisation breaks down
-    // is run again with twice the ILU level of fill (up to kMaxEscalations
+    // A request whose solve ends far from the tolerance (relative residual
+    // above kEscalateAbove * rtol: the preconditioner failed, not FP64's
+    // attainable accuracy) or whose factorisation or GMRES breaks down is
+    // run again with twice the ILU level of fill (up to kMaxEscalations
     // times): on thin shells a higher level converges where ILU(12) does not
     int st = GF_OK;
     for (int attempt = 0;; attempt++) {
@@ -118,8 +121,10 @@ void worker_main(gf_server_s* s) {
         st = gf_problem_solve(p, req.loads.data(), res.x.data(), 0, &res.info);
       }
       if (p) gf_problem_destroy(p);
-      const bool numerical = st == GF_ERR_NO_CONVERGENCE ||
-                             st == GF_ERR_ZERO_PIVOT || st == GF_ERR_BREAKDOWN;
+      const bool numerical =
+          (st == GF_ERR_NO_CONVERGENCE &&
+           !(res.info.rel_resid <= kEscalateAbove * req.opt.rtol)) ||
+          st == GF_ERR_ZERO_PIVOT || st == GF_ERR_BREAKDOWN;
       if (!numerical || attempt >= kMaxEscalations) break;
       const size_t before = req.bytes;
       req.opt.lev_fill = std::min(2 * std::max(req.opt.lev_fill, 1), 64);
