@@ -44,6 +44,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+ASM_FLOPS_PER_ELEM = 2 * 24 * 24 * 9 * 4  # 41,472
+
 # precision modes of gf_options.precision
 PREC_NAMES = {0: "fp64", 1: "mixed", 2: "mixed_pc"}
 
@@ -54,6 +57,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nx", type=int, default=400)
+    # other workloads for ad-hoc runs (the JSON line names the workload):
+    # cylinder R = 1 m, L = 3 m, nc x 2nc quads, t U[1,8] mm, clamped end,
+    # 50 kPa + 10 kN/m; wingbox (seeded, ~489k nodes at --wb-scale 1)
+    ap.add_argument("--mesh", default="plate",
+                    choices=["plate", "cylinder", "wingbox"])
+    ap.add_argument("--cyl-nc", type=int, default=256)
+    ap.add_argument("--wb-scale", type=float, default=1.0)
     ap.add_argument("--precision", default="both",
                     choices=["all", "both", "fp64", "mixed", "mixed_pc"])
     ap.add_argument("--reorder", default="amd", choices=["natural", "rcm", "amd"])
@@ -64,8 +74,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=5)
     # served request mix (separate JSON line; see serve_main)
     ap.add_argument("--serve", action="store_true")
-    ap.add_argument("--requests", type=int, default=32)
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--rate", type=float, default=0.0,
                     help="Poisson arrival rate (requests/s); 0: all at t=0")
@@ -146,10 +156,23 @@ def main():
     torch.cuda.init()
     nx = args.nx
     t0 = time.time()
-    mesh = M.plate(nx, nx, 1.0, 1.0, E=70e9, nu=0.3, thick=0.005, bc="ss",
-                   load="sine", load_mag=1e3)
-    print(f"[bench] mesh {nx}x{nx}: {mesh.nnodes} nodes in {time.time()-t0:.1f}s",
-          file=sys.stderr, flush=True)
+    if args.mesh == "cylinder":
+        nc = args.cyl_nc
+        mesh = M.cylinder(nc, 2 * nc, seed=args.seed)
+        wl = (f"cylinder R=1m L=3m {nc}x{2 * nc} MITC4, t U[1,8] mm (seed "
+              f"{args.seed}), clamped, 50 kPa + 10 kN/m")
+    elif args.mesh == "wingbox":
+        f = args.wb_scale
+        mesh = M.wingbox(nc=max(2, round(120 * f)), nh=max(2, round(16 * f)),
+                         nsb=max(1, round(30 * f)), seed=args.seed)
+        wl = (f"wingbox {mesh.name} (seed {args.seed}), panel t U[2,20] mm, "
+              f"root clamped, elliptic pressure 20 kPa")
+    else:
+        mesh = M.plate(nx, nx, 1.0, 1.0, E=70e9, nu=0.3, thick=0.005, bc="ss",
+                       load="sine", load_mag=1e3)
+        wl = f"plate {nx}x{nx} MITC4, t=5mm, SS, sine pressure"
+    print(f"[bench] mesh {mesh.name}: {mesh.nnodes} nodes in "
+          f"{time.time()-t0:.1f}s", file=sys.stderr, flush=True)
     ctx = G.Context(0)
     ev = HipEvents()
     stream = ctx.stream
@@ -315,8 +338,7 @@ def main():
                             "mixed_pc": "FP32 ILU and sweeps inside FP64 "
                                         "GMRES (FP64 SpMV, Arnoldi)"},
         "data": "synthetic",
-        "config": {"workload": f"plate {nx}x{nx} MITC4, t=5mm, SS, sine "
-                               f"pressure; {args.reorder} order + "
+        "config": {"workload": f"{wl}; {args.reorder} order + "
                                f"ILU({args.lev_fill}), GMRES({args.restart}) "
                                f"to 1e-8",
                    "nodes": mesh.nnodes, "dof": 6 * mesh.nnodes,
@@ -336,7 +358,17 @@ def main():
                               "achieved": asm_gbs, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": asm_gbs / HBM_PEAK_GBS,
                               "bytes_per_launch": asm_bytes,
-                              "launch_us": asm_ms * 1e3},
+                              "launch_us": asm_ms * 1e3,
+                              # the kernel is FP64-issue bound: the products
+                              # K_e += B_q^T D_q as implemented (24 x 24
+                              # entries x 9 strains x 4 points, one FMA each)
+                              "fp64": {
+                                  "flops_per_element": ASM_FLOPS_PER_ELEM,
+                                  "achieved_tflops": ASM_FLOPS_PER_ELEM
+                                  * mesh.nelems / (asm_ms * 1e-3) / 1e12,
+                                  "peak_tflops": FP64_PEAK_TFLOPS,
+                                  "frac": ASM_FLOPS_PER_ELEM * mesh.nelems
+                                  / (asm_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -216,6 +216,121 @@ def cylinder(nc, na, R=1.0, L=3.0, E=70e9, nu=0.3, t_range=(1e-3, 8e-3),
                 meta=dict(nc=nc, na=na, R=R, L=L, seed=seed, bc=bc))
 
 
+def wingbox(nc=120, nh=16, nsb=30, nribs=50, span=30.0, c_root=6.0,
+            c_tip=1.5, h_frac=0.12, E=70e9, nu=0.3, t_range=(2e-3, 20e-3),
+            seed=0, p_root=20e3):
+    """Synthetic tapered wingbox from structured quad patches that share
+    their edge nodes.
+
+    The span runs along y in [0, span]; the chord c(y) tapers linearly from
+    c_root to c_tip between the front spar (x = 0) and the rear spar
+    (x = c(y)); the box height is h(y) = h_frac c(y) with the lower and upper
+    skins at z = -h/2 and +h/2.  ``nribs`` ribs at equal spacing (the first
+    at the root, the last at the tip) divide the span into nribs - 1 bays of
+    ``nsb`` elements; skins have ``nc`` elements along the chord, spars and
+    ribs ``nh`` through the height.  Every panel (one bay of a skin or a
+    spar, or one rib) has its own thickness, uniform in ``t_range`` from
+    ``seed`` (numpy PCG64).  The root (y = 0) is clamped (all 6 DOFs); an
+    upward elliptic pressure p(y) = p_root sqrt(1 - (y/span)^2) acts on the
+    lower skin as consistent nodal loads.  The defaults give 489,362 nodes
+    and 495,840 elements.
+    """
+    Ns = (nribs - 1) * nsb
+    y = span * np.arange(Ns + 1) / Ns
+    c = c_root + (c_tip - c_root) * y / span
+    h = h_frac * c
+    nskin = (nc + 1) * (Ns + 1)
+    nspar = (nh - 1) * (Ns + 1)
+    nrib = (nc - 1) * (nh - 1)
+    offU, offF = nskin, 2 * nskin
+    offR = offF + nspar
+    offB = offR + nspar
+    nnodes = offB + nribs * nrib
+
+    def skin(i, j, upper):
+        return (offU if upper else 0) + i + (nc + 1) * j
+
+    def spar(k, j, rear):
+        k, j = np.broadcast_arrays(k, j)
+        i = nc if rear else 0
+        out = (offR if rear else offF) + (k - 1) + (nh - 1) * j
+        out = np.where(k == 0, skin(i, j, False), out)
+        return np.where(k == nh, skin(i, j, True), out)
+
+    def rib(r, i, k):
+        i, k = np.broadcast_arrays(i, k)
+        jr = r * nsb
+        out = offB + r * nrib + (i - 1) + (nc - 1) * (k - 1)
+        out = np.where(i == 0, spar(k, jr, False), out)
+        out = np.where(i == nc, spar(k, jr, True), out)
+        out = np.where(k == 0, skin(i, jr, False), out)
+        return np.where(k == nh, skin(i, jr, True), out)
+
+    xpts = np.zeros((nnodes, 3))
+    I, J = np.meshgrid(np.arange(nc + 1), np.arange(Ns + 1), indexing="ij")
+    for upper in (False, True):
+        idx = skin(I, J, upper)
+        xpts[idx, 0] = c[J] * I / nc
+        xpts[idx, 1] = y[J]
+        xpts[idx, 2] = (0.5 if upper else -0.5) * h[J]
+    K, J = np.meshgrid(np.arange(1, nh), np.arange(Ns + 1), indexing="ij")
+    for rear in (False, True):
+        idx = spar(K, J, rear)
+        xpts[idx, 0] = c[J] if rear else 0.0
+        xpts[idx, 1] = y[J]
+        xpts[idx, 2] = h[J] * (K / nh - 0.5)
+    I, K = np.meshgrid(np.arange(1, nc), np.arange(1, nh), indexing="ij")
+    for r in range(nribs):
+        jr = r * nsb
+        idx = rib(r, I, K)
+        xpts[idx, 0] = c[jr] * I / nc
+        xpts[idx, 1] = y[jr]
+        xpts[idx, 2] = h[jr] * (K / nh - 0.5)
+
+    conns, panels = [], []
+    # skins: lower then upper, panel = bay
+    I, J = np.meshgrid(np.arange(nc), np.arange(Ns), indexing="ij")
+    for s_id, upper in enumerate((False, True)):
+        q = [skin(I, J, upper), skin(I + 1, J, upper), skin(I, J + 1, upper),
+             skin(I + 1, J + 1, upper)]
+        conns.append(np.stack([a.ravel() for a in q], 1))
+        panels.append((s_id * (nribs - 1) + J // nsb).ravel())
+    nlower = nc * Ns
+    # spars: front then rear, panel = bay
+    K, J = np.meshgrid(np.arange(nh), np.arange(Ns), indexing="ij")
+    for s_id, rear in enumerate((False, True)):
+        q = [spar(K, J, rear), spar(K, J + 1, rear), spar(K + 1, J, rear),
+             spar(K + 1, J + 1, rear)]
+        conns.append(np.stack([a.ravel() for a in q], 1))
+        panels.append(((2 + s_id) * (nribs - 1) + J // nsb).ravel())
+    # ribs: panel = rib
+    I, K = np.meshgrid(np.arange(nc), np.arange(nh), indexing="ij")
+    for r in range(nribs):
+        q = [rib(r, I, K), rib(r, I + 1, K), rib(r, I, K + 1),
+             rib(r, I + 1, K + 1)]
+        conns.append(np.stack([a.ravel() for a in q], 1))
+        panels.append(np.full(nc * nh, 4 * (nribs - 1) + r))
+    conn = np.concatenate(conns).astype(np.int32)
+    panel = np.concatenate(panels)
+    rng = np.random.default_rng(seed)
+    t_panel = rng.uniform(t_range[0], t_range[1], size=5 * nribs - 4)
+    data = _data(len(conn), E, nu, 1.0)
+    data[:, 2] = t_panel[panel]
+    root = np.flatnonzero(np.abs(xpts[:, 1]) < 1e-12 * span)
+    bcs = (6 * root[:, None] + np.arange(6)[None, :]).ravel()
+    up = lambda x: np.tile(np.array([0.0, 0.0, 1.0]), (len(x), 1))
+    loads = consistent_pressure_loads(
+        conn[:nlower], xpts,
+        lambda x: p_root * np.sqrt(np.maximum(0.0, 1.0 - (x[:, 1] / span) ** 2)),
+        normal_fn=up)
+    loads[bcs] = 0.0
+    return Mesh(conn, xpts, data, bcs.astype(np.int32), loads,
+                name=f"wingbox{nc}x{nh}x{Ns}r{nribs}",
+                meta=dict(nc=nc, nh=nh, nsb=nsb, nribs=nribs, span=span,
+                          c_root=c_root, c_tip=c_tip, h_frac=h_frac, seed=seed,
+                          panels=5 * nribs - 4))
+
+
 def served_requests(n, seed=0, scale=1.0):
     """The served request mix: seeded plates (n x n quads, n log-uniform in
     [32, 256]) and cylinders (nc x na from 64 x 128 to 192 x 384,

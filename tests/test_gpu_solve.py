@@ -56,6 +56,9 @@ CASES = [
      lambda: M.plate(24, 24, thick=0.005, bc="ss", load="sine"), 1, 3, 50, 1e-8),
     ("cylinder_amd_ilu2",
      lambda: M.cylinder(24, 16, seed=11), 2, 2, 50, 1e-8),
+    # skins, spars and ribs meeting at shared edge nodes (T-junctions)
+    ("wingbox_amd_ilu3",
+     lambda: M.wingbox(nc=10, nh=3, nsb=2, nribs=4, seed=5), 2, 3, 50, 1e-8),
 ]
 
 
