@@ -309,6 +309,13 @@ int KrylovWork::dots(const double* Vb, size_t ldv, int nvec, const double* x,
   return check_launch(ctx, "multi_dot");
 }
 
+#ifndef GPUFEM_GMRES_CYCLE_TARGET
+#define GPUFEM_GMRES_CYCLE_TARGET 0.1
+#endif
+// each restart cycle runs until its residual estimate falls below this
+// fraction of rtol ||b|| (see the comment at its use)
+constexpr double kCycleTarget = GPUFEM_GMRES_CYCLE_TARGET;
+
 int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
                 int max_iters, int stall_cycles, double rtol, int prec,
                 GmresResult* res) {
@@ -363,8 +370,8 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     // cycles aim at 0.1 rtol |b|: the true residual of an ill-conditioned
     // preconditioned system trails the Arnoldi estimate, and cycles that stop
     // exactly at rtol |b| restart again and again just above it
-    const double tol_abs =
-        ir ? std::max(1e-5 * beta, 0.1 * rtol * bnorm) : 0.1 * rtol * bnorm;
+    const double tol_abs = ir ? std::max(1e-5 * beta, kCycleTarget * rtol * bnorm)
+                              : kCycleTarget * rtol * bnorm;
     hipLaunchKernelGGL(scale_by_inv, grid1(N), dim3(256), 0, s, N, W.r, W.nrm,
                        W.V);
     hipLaunchKernelGGL(init_g, dim3(1), dim3(64 * div_up(m + 1, 64)), 0, s, m,
