@@ -115,6 +115,26 @@ class HipEvents:
         return ms.value
 
 
+def pmc_traffic(args, nodes, nnzb_ilu):
+    """HBM bytes per sweep launch measured with rocprofv3 PMC counters
+    (FETCH_SIZE and WRITE_SIZE in two runs, scaled by the same counters on a
+    1 GiB device copy) for this workload, read from the summary committed in
+    profiles/; None when no summary matches (nodes and ILU blocks)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "pmc_sweep_plate400_amd_ilu12.md")
+    if args.mesh != "plate" or not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("traffic_") and ":" in line:
+            k, v = line.split(":", 1)
+            vals[k.strip()] = float(v.split()[0])
+    if (vals.get("traffic_nodes") != nodes or
+            vals.get("traffic_nnzb_ilu") != nnzb_ilu):
+        return None
+    return vals.get("traffic_bytes_per_launch")
+
+
 def sweep_bytes(mrp, mcp, w):
     """Algorithmic bytes of one L and one U sweep (values + column indices of
     the triangle incl. the diagonal block, row pointers, rhs read, solution
@@ -348,7 +368,8 @@ def main():
                   for k, v in results.items()},
         "roofline": {"bound": "hbm", "kernel": "trsv_syncfree_kernel (L and U)",
                      "achieved": sweep_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": sweep_gbs / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic(args, nn, nm),
                      "bytes_per_launch": sweep_bytes_launch,
                      "launch_us": sweep_launch_s * 1e6,
                      "levels": [info_lv["levels_lower"], info_lv["levels_upper"]]},

@@ -16,6 +16,9 @@ struct KrylovWork {
   double *h = nullptr, *h2 = nullptr, *nrm = nullptr, *y = nullptr;
   double *state = nullptr, *partial = nullptr, *scal = nullptr;
   int* gate = nullptr;  // second Gram-Schmidt pass needed (device flag)
+  // DCGS2: Hessenberg before rotations (column-major, (m+1) x (m+1)), the
+  // step's dot products, update coefficients, provisional subdiagonal
+  double *hc = nullptr, *dd = nullptr, *co = nullptr, *rho = nullptr;
   float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
   double* host_scal = nullptr;  // pinned: two (estimate, h_{j+1,j}) slots
   hipEvent_t ev[2] = {nullptr, nullptr};

@@ -335,7 +335,7 @@ extern "C" int orc_gmres(int n, const int* krowp, const int* kcols,
   // Each outer step: true residual r = b - A x in FP64, then one GMRES(m)
   // cycle on A d = r started from d = 0, then x += d.  In FP64 mode this is
   // plain restarted GMRES(m) with the cycle stopping when its residual
-  // estimate reaches 0.1 rtol |b| (aiming below the tolerance: on
+  // estimate reaches 0.3 rtol |b| (aiming below the tolerance: on
   // ill-conditioned preconditioned systems the true residual trails the
   // estimate, and a cycle that stops exactly at rtol |b| leaves the true
   // residual just above it, restarting again and again); in mixed mode the
@@ -352,8 +352,8 @@ extern "C" int orc_gmres(int n, const int* krowp, const int* kcols,
     }
     if (total_it >= max_iters) break;
     outer++;
-    double tol_abs = mixed ? std::max(1e-5 * beta, 0.1 * rtol * bnorm)
-                           : 0.1 * rtol * bnorm;
+    double tol_abs = mixed ? std::max(1e-5 * beta, 0.3 * rtol * bnorm)
+                           : 0.3 * rtol * bnorm;
     for (size_t i = 0; i < N; i++) V[0][i] = r[i] / beta;
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
