@@ -22,12 +22,15 @@ element colouring) resident in HBM: Jacobian assembly + boundary conditions
 device.  The JSON line reports the fastest converged precision as
 ``value``; every mode run is listed under ``paths``.
 
-Extra fields: ``roofline`` (the dominant kernel, measured with HIP events on
-the library's stream), ``assembly_roofline``, ``cpu_baseline`` (the CPU
-oracle on the host cores, bounded sample, see ``sample``) and
-``latency_ms`` p50/p99 over the timed steps.  ``--serve`` runs the served
-request mix instead (its own JSON line); ``--profile-kernels`` a short run
-of the hot kernels for rocprofv3.
+Extra fields: ``roofline`` (the dominant kernel, the sweeps, measured with
+HIP events on the library's stream; ``traffic`` is the HBM bytes per launch
+measured with rocprofv3 PMC counters for this workload and committed in
+profiles/, or null for other workloads), ``assembly_roofline`` (with the
+FP64 rate of the element products), ``cpu_baseline`` (the CPU oracle on the
+host cores, bounded sample, see ``sample``) and ``latency_ms`` p50/p99 over
+the timed steps.  ``--mesh cylinder|wingbox`` runs the other workloads;
+``--serve`` runs the served request mix instead (its own JSON line);
+``--profile-kernels`` a short run of the hot kernels for rocprofv3.
 """
 from __future__ import annotations
 
