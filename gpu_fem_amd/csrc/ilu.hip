@@ -72,6 +72,7 @@ __global__ void __launch_bounds__(kIluThreads)
   __shared__ int pos[2][kMaxRowBlocks];  // positions for this k and the next
   __shared__ int ub[2][2];               // U-part bounds [u0, u1) per buffer
   __shared__ T udg[2][36];               // U_kk per buffer
+  __shared__ T udi[2][6];                // 1 / diagonal of U_kk per buffer
   __shared__ T lik[36];
   const int li = blockIdx.x;
   if (li >= nrows) return;
@@ -90,7 +91,11 @@ __global__ void __launch_bounds__(kIluThreads)
       ub[buf][0] = u0;
       ub[buf][1] = u1;
     }
-    if (tid < 36) udg[buf][tid] = vals[36 * (size_t)diag[k] + tid];
+    if (tid < 36) {
+      const T u = vals[36 * (size_t)diag[k] + tid];
+      udg[buf][tid] = u;
+      if (tid % 7 == 0) udi[buf][tid / 7] = T(1) / u;
+    }
     for (int t = tid; t < u1 - u0; t += blockDim.x) {
       const int j = cols[u0 + t];
       int lo = kk + 1 - r0, hi = r1 - r0;
@@ -110,6 +115,7 @@ __global__ void __launch_bounds__(kIluThreads)
     const int cur = (kk - r0) & 1;
     T* Aik = vals + 36 * (size_t)kk;
     const T* Ukk = udg[cur];
+    const T* Ui = udi[cur];
     // L_ik = A_ik U_kk^-1 (row r by forward substitution over columns) on
     // lanes 0-5, the next k staged by the other lanes meanwhile
     if (tid < 6) {
@@ -120,7 +126,7 @@ __global__ void __launch_bounds__(kIluThreads)
         T v = Aik[6 * r + c];
 #pragma unroll
         for (int m = 0; m < c; m++) v -= x[m] * Ukk[6 * m + c];
-        x[c] = v / Ukk[6 * c + c];
+        x[c] = v * Ui[c];
       }
 #pragma unroll
       for (int c = 0; c < 6; c++) {
@@ -136,7 +142,11 @@ __global__ void __launch_bounds__(kIluThreads)
         ub[cur ^ 1][0] = u0;
         ub[cur ^ 1][1] = u1;
       }
-      if (tl < 36) udg[cur ^ 1][tl] = vals[36 * (size_t)diag[k] + tl];
+      if (tl < 36) {
+        const T u = vals[36 * (size_t)diag[k] + tl];
+        udg[cur ^ 1][tl] = u;
+        if (tl % 7 == 0) udi[cur ^ 1][tl / 7] = T(1) / u;
+      }
       for (int t = tl; t < u1 - u0; t += nth) {
         const int j = cols[u0 + t];
         int lo = kk + 2 - r0, hi = r1 - r0;
@@ -241,6 +251,7 @@ __global__ void __launch_bounds__(kIluThreads)
   extern __shared__ __align__(16) unsigned char ilu_dyn[];
   __shared__ int pos[2][kMaxRowBlocks];  // local positions in row i, or -1
   __shared__ T udg[2][36];
+  __shared__ T udi[2][6];  // 1 / diagonal of U_kk
   __shared__ T lik[36];
   const int li = blockIdx.x;
   if (li >= nrows) return;
@@ -314,13 +325,20 @@ __global__ void __launch_bounds__(kIluThreads)
     }
     for (int t = ts + kIluCols * ns; t < nu; t += ns)
       pos[buf][t] = find(cols[u0 + t], kl + 1);
-    if (ts < 36) udg[buf][ts] = pd;
+    if (ts < 36) {
+      udg[buf][ts] = pd;
+      if (ts % 7 == 0) udi[buf][ts / 7] = T(1) / pd;
+    }
   };
   if (dl > 0) {
     // pivot 0 staged directly by all threads
     const int k = scol[0];
     const int u0 = ubs[0], nu = ubs[1] - u0;
-    if (tid < 36) udg[0][tid] = vals[36 * (size_t)diag[k] + tid];
+    if (tid < 36) {
+      const T u = vals[36 * (size_t)diag[k] + tid];
+      udg[0][tid] = u;
+      if (tid % 7 == 0) udi[0][tid / 7] = T(1) / u;
+    }
     for (int t = tid; t < nu; t += nth) pos[0][t] = find(cols[u0 + t], 1);
     load_u(0, uv);
     load_cols(1);
@@ -333,6 +351,7 @@ __global__ void __launch_bounds__(kIluThreads)
     if (tid < 6) {
       const int r = tid;
       const T* Ukk = udg[cur];
+      const T* Ui = udi[cur];
       T* Aik = row + 36 * kl;
       T x[6];
 #pragma unroll
@@ -340,7 +359,7 @@ __global__ void __launch_bounds__(kIluThreads)
         T v = Aik[6 * r + c];
 #pragma unroll
         for (int m = 0; m < c; m++) v -= x[m] * Ukk[6 * m + c];
-        x[c] = v / Ukk[6 * c + c];
+        x[c] = v * Ui[c];
       }
 #pragma unroll
       for (int c = 0; c < 6; c++) {
@@ -464,9 +483,10 @@ __global__ void __launch_bounds__(256)
       if (active && r > c) s -= D[c] * yc;
     }
   } else {
+    const T dinv = active ? T(1) / D[r] : T(0);
 #pragma unroll
     for (int c = 5; c >= 0; c--) {
-      if (active && r == c) s /= D[c];
+      if (active && r == c) s *= dinv;
       const T yc = __shfl(s, base + c, 64);
       if (active && r < c) s -= D[c] * yc;
     }
@@ -614,6 +634,12 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
 #pragma unroll
       for (int mm = 0; mm < 6; mm++)
         drow[mm] = (live && gl < 6) ? lu[36 * (size_t)di + 6 * gl + mm] : T(0);
+      // reciprocal of this lane's diagonal entry, formed before any wait
+      T dd = T(1);
+#pragma unroll
+      for (int mm = 0; mm < 6; mm++)
+        if (gl == mm) dd = drow[mm];
+      const T dinv = (!kLower && live && gl < 6) ? T(1) / dd : T(0);
       // Per pass of up to kGather blocks: the first kSweepIter*kS blocks'
       // values are loaded before anything waits; the pass's column indices
       // are read (lane q: blocks q, q+G, ...) and the dependencies' entries
@@ -747,7 +773,7 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
       } else {
 #pragma unroll
         for (int mm = 5; mm >= 0; mm--) {
-          if (gl == mm) s /= drow[mm];
+          if (gl == mm) s *= dinv;
           const T ym = __shfl(s, g0 + mm, 64);
           if (gl < 6 && gl < mm) s -= drow[mm] * ym;
         }

@@ -392,8 +392,11 @@ __global__ void __launch_bounds__(256)
   double cc = 0.0;
   double* hj = Hc + (size_t)j * ldh;
   for (int i = tid; i <= j; i += 256) {
+    // all lanes walk k together so each load is one coalesced row segment;
+    // entries below the subdiagonal (k < i - 1) are skipped, not read
     double hs = 0.0;
-    for (int k = (i > 0 ? i - 1 : 0); k < j; k++) hs += Hc[(size_t)k * ldh + i] * ls[k];
+    for (int k = 0; k < j; k++)
+      if (k + 1 >= i) hs += Hc[(size_t)k * ldh + i] * ls[k];
     const double ci = i < j ? t[i] : cj;
     hj[i] = (ci - hs) * inva;
     co[m + 1 + i] = ci;
