@@ -62,7 +62,10 @@ constexpr int kMaxRowBlocks = 1024;
 // indices are staged in LDS (positions of U(k) in row i are found by binary
 // search there), L_ik is kept in LDS for the update, and the next k's
 // pattern bounds are loaded while the current update runs.
-constexpr int kIluThreads = 256;
+#ifndef GPUFEM_ILU_THREADS
+#define GPUFEM_ILU_THREADS 256
+#endif
+constexpr int kIluThreads = GPUFEM_ILU_THREADS;
 template <typename T>
 __global__ void __launch_bounds__(kIluThreads)
     ilu_row_kernel(const int* __restrict__ level_rows, int nrows,
@@ -240,7 +243,10 @@ __global__ void __launch_bounds__(kIluThreads)
 // columns in row i are searched.  The row is written back once at the end.
 // Arithmetic is that of ilu_row_kernel, operation for operation.
 // Dynamic LDS: 36 * nb values, nb column indices, 2 * nl pivot bounds.
-constexpr int kIluPairs = 4;  // (block, column) pairs of U_kj per thread
+#ifndef GPUFEM_ILU_PAIRS
+#define GPUFEM_ILU_PAIRS 4
+#endif
+constexpr int kIluPairs = GPUFEM_ILU_PAIRS;  // (block, column) pairs of U_kj per thread
 constexpr int kIluCols = 4;   // prefetched column indices per staging thread
 template <typename T>
 __global__ void __launch_bounds__(kIluThreads)
