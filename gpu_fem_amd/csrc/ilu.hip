@@ -62,8 +62,12 @@ constexpr int kMaxRowBlocks = 1024;
 // indices are staged in LDS (positions of U(k) in row i are found by binary
 // search there), L_ik is kept in LDS for the update, and the next k's
 // pattern bounds are loaded while the current update runs.
+// 512 threads with two register pairs each measured fastest on the 400x400
+// plate, AMD + ILU(16): 293 ms a factorisation (256 x 4: 311, 256 x 2: 306,
+// 128 x 4: 350, 128 x 2: 366).  Forming the next pivot's L_ik while the
+// current update runs (one more barrier a pivot) measured slower (312 ms).
 #ifndef GPUFEM_ILU_THREADS
-#define GPUFEM_ILU_THREADS 256
+#define GPUFEM_ILU_THREADS 512
 #endif
 constexpr int kIluThreads = GPUFEM_ILU_THREADS;
 template <typename T>
@@ -244,7 +248,7 @@ __global__ void __launch_bounds__(kIluThreads)
 // Arithmetic is that of ilu_row_kernel, operation for operation.
 // Dynamic LDS: 36 * nb values, nb column indices, 2 * nl pivot bounds.
 #ifndef GPUFEM_ILU_PAIRS
-#define GPUFEM_ILU_PAIRS 4
+#define GPUFEM_ILU_PAIRS 2
 #endif
 constexpr int kIluPairs = GPUFEM_ILU_PAIRS;  // (block, column) pairs of U_kj per thread
 constexpr int kIluCols = 4;   // prefetched column indices per staging thread
