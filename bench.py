@@ -85,6 +85,8 @@ def parse():
                     help="Poisson arrival rate (requests/s); 0: all at t=0")
     ap.add_argument("--serve-precision", type=int, default=2)
     ap.add_argument("--serve-lev-fill", type=int, default=12)
+    ap.add_argument("--serve-stall", type=int, default=4,
+                    help="stall_cycles of the served requests")
     ap.add_argument("--seed", type=int, default=0)
     # short kernel-only run for profilers (rocprofv3 --pmc / --kernel-trace)
     ap.add_argument("--profile-kernels", action="store_true")
@@ -165,8 +167,8 @@ def main():
         # one hardware queue per worker stream (HIP's default is 4), set
         # before the HIP runtime starts
         q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
-        if q < min(args.slots, 16):
-            os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.slots, 16))
+        if q < min(args.slots, 32):
+            os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.slots, 32))
         serve_main(args)
         return
     if args.profile_kernels:
@@ -467,11 +469,11 @@ def serve_main(args):
           file=sys.stderr, flush=True)
     # AMD + ILU(12): AMD + ILU(3) stagnates near a relative residual of 1 on
     # the thinner plates and cylinders of this mix
-    # stall_cycles: a solve whose residual falls by less than 10% over 4
-    # restart cycles stops early (and is re-run with twice the level of fill
-    # when it stopped far from the tolerance)
+    # stall_cycles: a solve whose residual falls by less than 10% over
+    # --serve-stall restart cycles stops early (and is re-run with twice the
+    # level of fill when it stopped far from the tolerance)
     opts = dict(reorder=2, lev_fill=args.serve_lev_fill, restart=200,
-                rtol=1e-8, max_iters=20000, stall_cycles=4)
+                rtol=1e-8, max_iters=20000, stall_cycles=args.serve_stall)
     ctx = G.Context(0)
     # plain FP64, one at a time
     plain_x, plain_lat, plain_ok = {}, [], 0

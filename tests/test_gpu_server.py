@@ -83,3 +83,31 @@ def test_budget_admission(gpu_ctx):
     big = M.plate(300, 300)
     assert srv.submit(99, big, **OPTS) == -7
     srv.close()
+
+
+def test_stagnating_request_is_rerun_with_more_fill(gpu_ctx):
+    """A request whose ILU(1) solve stagnates (free-edge plate, relative
+    residual ~0.9) is re-run by the server with ILU(2), converges, and equals
+    the same request solved alone with ILU(2)."""
+    print("[gpu] server escalation ILU(1) -> ILU(2)", flush=True)
+    m = M.plate(16, 16, 2.0, 1.0, E=70e9, nu=0.3, thick=0.005, bc="ss",
+                load="nodal_sine", load_mag=1.0)
+    opts = dict(reorder=1, restart=100, rtol=1e-9, max_iters=4000,
+                stall_cycles=3)
+    p = G.Problem(gpu_ctx, m, lev_fill=2, **opts)
+    p.assemble()
+    p.factor()
+    x2, info2 = p.solve()
+    p.close()
+    assert info2.status == 0
+    srv = G.Server(budget_gb=200.0, slots=2)
+    assert srv.submit(0, m, lev_fill=1, **opts) == 0
+    srv.drain()
+    r = srv.poll()
+    st = srv.stats()
+    srv.close()
+    assert r is not None
+    _, x, info, _ = r
+    assert info.status == 0 and info.rel_resid <= 1e-9
+    assert st["escalated"] == 1 and st["completed"] == 1
+    assert np.linalg.norm(x - x2) <= 1e-10 * np.linalg.norm(x2)

@@ -147,3 +147,31 @@ def _copy_dev(addr, count, dtype):
     st = hip.hipMemcpy(out.ctypes.data, addr, out.nbytes, 2)  # D2H
     assert st == 0
     return out
+
+
+def _free_edge_plate():
+    # 2 m x 1 m, t = 5 mm, simply supported with a free edge at y = Ly: with
+    # ILU(0) or ILU(1) GMRES stagnates near a relative residual of 0.9 (the
+    # oracle agrees), with RCM + ILU(2) it converges
+    return M.plate(16, 16, 2.0, 1.0, E=70e9, nu=0.3, thick=0.005, bc="ss",
+                   load="nodal_sine", load_mag=1.0)
+
+
+def test_stall_cycles_stop_a_stagnating_solve(gpu_ctx):
+    say("[gpu] stagnation stop (stall_cycles) on the free-edge plate, ILU(0)")
+    mesh = _free_edge_plate()
+    its = {}
+    for stall in (0, 3):
+        p = G.Problem(gpu_ctx, mesh, reorder=0, lev_fill=0, restart=20,
+                      rtol=1e-9, max_iters=1500, stall_cycles=stall)
+        p.assemble()
+        p.factor()
+        x, info = p.solve(raise_on_error=False)
+        msg = G.lib().gf_context_last_error(gpu_ctx.h).decode()
+        p.close()
+        assert info.status == -6 and info.rel_resid > 1e-3
+        its[stall] = (info.iterations, info.outer, msg)
+    # without the option the solve runs to its iteration limit, as the
+    # reference's does; with it, it stops after a handful of cycles
+    assert its[0][0] == 1500 and "iteration limit" in its[0][2]
+    assert its[3][0] < 1500 and its[3][1] <= 20 and "stagnated" in its[3][2]
