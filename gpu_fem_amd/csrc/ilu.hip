@@ -9,8 +9,15 @@
 // sweep a non-unit one.
 //
 // Schedules
-//   factorisation: rows grouped by dependency level on the host (the levels
-//     of the factorisation equal those of the L sweep); one launch per level.
+//   factorisation: rows ordered by dependency level on the host (the levels
+//     of the factorisation equal those of the L sweep).  Default (dataflow):
+//     a few launches, each over a run of levels, in which a workgroup takes
+//     the next row in level order from a counter and waits, pivot by pivot,
+//     only for the pivot rows it is about to use (a per-row completion flag,
+//     released after the row is written); no level waits for the whole
+//     previous level.  GPUFEM_ILU_SCHEDULE=level runs one launch per level
+//     instead.  The arithmetic of a row is the same either way, so the two
+//     schedules give bitwise-identical factors.
 //   sweeps: one persistent launch per sweep, synchronised through the data:
 //     the output vector is pre-filled with a signalling-NaN sentinel, each
 //     entry is written once with a relaxed agent-scope atomic store, and a
@@ -47,16 +54,114 @@ struct gf_ilu_s {
   int nlchunk = 0, nuchunk = 0;
   bool lchains = false, uchains = false;  // chunks are chains in row order
   unsigned long long* counter = nullptr;   // chunk dispenser (monotonic)
-  int* err = nullptr;                      // set when a bounded wait expires
+  int* err = nullptr;  // set when a bounded wait expires: [0] sweeps,
+                      // [1] factorisation
   unsigned long long base = 0;             // counter value at sweep start
   int grid = 0;                            // workgroups per sweep launch
   bool level_sweeps = false;
+  // dataflow factorisation: done[i] == epoch once row i is final
+  int* done = nullptr;
+  int epoch = 0;
 };
 
 namespace gf {
 namespace {
 
 constexpr int kMaxRowBlocks = 1024;
+
+// Dataflow schedule of the factorisation (kFlow kernels).  Every workgroup
+// takes exactly one ticket from the counter; ticket t factors
+// level_rows[t].  Tickets follow level order, so the pivot rows a row waits
+// for were handed out before it to workgroups that are already running, and
+// the lowest unfinished ticket never waits: the schedule cannot deadlock.
+struct Flow {
+  unsigned long long* counter = nullptr;
+  unsigned long long base = 0;  // counter value at the launch
+  int* done = nullptr;          // done[row] == epoch: row is final
+  int epoch = 0;
+  int* err = nullptr;           // set when a bounded wait expires
+};
+constexpr unsigned kFlowSpinLimit = 1u << 23;  // polls per wait (seconds)
+
+// The row this workgroup factors (dataflow) or blockIdx.x's row (levels).
+template <bool kFlow>
+__device__ __forceinline__ int flow_row(const int* level_rows, int nrows,
+                                        const Flow& f, int* s_row) {
+  if (!kFlow) return blockIdx.x < nrows ? level_rows[blockIdx.x] : -1;
+  if (threadIdx.x == 0) {
+    const long long t = (long long)(atomicAdd(f.counter, 1ull) - f.base);
+    const bool ok = t >= 0 && t < nrows;
+    *s_row = ok ? level_rows[t] : -1;
+    // each workgroup takes one ticket of [0, nrows): anything else means
+    // the counter was not where the host expected it
+    if (!ok)
+      __hip_atomic_store(f.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  return *s_row;
+}
+
+// Wait until the pivot rows piv[0..target] are final.  s_rdy[par] holds the
+// length of the prefix of piv known to be final; wavefront 0 extends it
+// 64 flags per load (one ballot), sleeping between polls while the next
+// pivot row is not done, then takes the agent-scope acquire that makes the
+// pivot rows' values visible, and publishes the new prefix in the other
+// slot, s_rdy[par ^ 1]: every thread reads s_rdy[par] to decide whether to
+// wait, so it must not change until all have passed the barrier.
+// Uniform: every thread of the workgroup calls it with the same target.
+__device__ __forceinline__ void flow_wait(const int* piv, int npiv,
+                                          int target, int* s_rdy, int& par,
+                                          const Flow& f) {
+  if (s_rdy[par] > target) return;
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int rdy = s_rdy[par];
+    unsigned spins = 0;
+    int nap = 1;
+    while (rdy <= target) {
+      const int t = rdy + lane;
+      const bool pend =
+          t < npiv && __hip_atomic_load(f.done + piv[t], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT) != f.epoch;
+      const unsigned long long m = __ballot(pend);
+      if (m == 0) {
+        rdy = min(rdy + 64, npiv);
+        continue;
+      }
+      rdy += __ffsll((long long)m) - 1;
+      if (rdy > target) break;
+      ++spins;
+      const bool failed =
+          spins > kFlowSpinLimit ||
+          (spins % 256 == 0 && __hip_atomic_load(f.err, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT));
+      if (failed) {
+        // give up: the factors are reported invalid, the grid drains
+        if (lane == 0)
+          __hip_atomic_store(f.err, 1, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        rdy = npiv;
+        break;
+      }
+      for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(1);
+      nap = nap * 2 > 8 ? 8 : nap * 2;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (lane == 0) s_rdy[par ^ 1] = rdy;
+  }
+  __syncthreads();
+  par ^= 1;
+}
+
+// Publish row i as final: every wavefront's stores of the row are released
+// at agent scope before the flag is set.
+__device__ __forceinline__ void flow_publish(int i, const Flow& f) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(f.done + i, f.epoch, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // One 256-thread workgroup per row of the current level.  The row's column
 // indices are staged in LDS (positions of U(k) in row i are found by binary
@@ -70,24 +175,32 @@ constexpr int kMaxRowBlocks = 1024;
 #define GPUFEM_ILU_THREADS 512
 #endif
 constexpr int kIluThreads = GPUFEM_ILU_THREADS;
-template <typename T>
+template <typename T, bool kFlow>
 __global__ void __launch_bounds__(kIluThreads)
     ilu_row_kernel(const int* __restrict__ level_rows, int nrows,
                    const int* __restrict__ rowp, const int* __restrict__ cols,
-                   const int* __restrict__ diag, T* vals, int* pivot) {
+                   const int* __restrict__ diag, T* vals, int* pivot,
+                   Flow f) {
   __shared__ int scol[kMaxRowBlocks];
   __shared__ int pos[2][kMaxRowBlocks];  // positions for this k and the next
   __shared__ int ub[2][2];               // U-part bounds [u0, u1) per buffer
   __shared__ T udg[2][36];               // U_kk per buffer
   __shared__ T udi[2][6];                // 1 / diagonal of U_kk per buffer
   __shared__ T lik[36];
-  const int li = blockIdx.x;
-  if (li >= nrows) return;
-  const int i = level_rows[li];
+  __shared__ int s_row, s_rdy[2];
+  const int i = flow_row<kFlow>(level_rows, nrows, f, &s_row);
+  if (i < 0) return;
   const int tid = threadIdx.x;
   const int r0 = rowp[i], r1 = rowp[i + 1], di = diag[i];
   for (int t = tid; t < r1 - r0; t += blockDim.x) scol[t] = cols[r0 + t];
+  if (tid == 0) s_rdy[0] = 0;
+  int par = 0;
   __syncthreads();
+  // dataflow: pivot row scol[kl] is waited for before it is staged (kl) or
+  // used (L_ik reads U_kk staged from it; the update reads U_kj)
+  auto wait_for = [&](int kl) {
+    if (kFlow) flow_wait(scol, di - r0, kl, s_rdy, par, f);
+  };
   // everything about pivot row k (its U part, U_kk, the positions of its
   // columns in row i) is final before this row starts, so it is staged for
   // the next k while the current k's update runs
@@ -116,10 +229,14 @@ __global__ void __launch_bounds__(kIluThreads)
       pos[buf][t] = (lo < r1 - r0 && scol[lo] == j) ? r0 + lo : -1;
     }
   };
-  if (r0 < di) stage(r0, 0);
+  if (r0 < di) {
+    wait_for(0);
+    stage(r0, 0);
+  }
   __syncthreads();
   for (int kk = r0; kk < di; kk++) {
     const int cur = (kk - r0) & 1;
+    if (kk + 1 < di) wait_for(kk + 1 - r0);
     T* Aik = vals + 36 * (size_t)kk;
     const T* Ukk = udg[cur];
     const T* Ui = udi[cur];
@@ -235,6 +352,7 @@ __global__ void __launch_bounds__(kIluThreads)
 #pragma unroll
     for (int r = 0; r < 6; r++) A[6 * r + c] = x[r];
   }
+  if (kFlow) flow_publish(i, f);
 }
 
 // The same factorisation with row i held in LDS for the whole row: L_ik is
@@ -252,20 +370,21 @@ __global__ void __launch_bounds__(kIluThreads)
 #endif
 constexpr int kIluPairs = GPUFEM_ILU_PAIRS;  // (block, column) pairs of U_kj per thread
 constexpr int kIluCols = 4;   // prefetched column indices per staging thread
-template <typename T>
+template <typename T, bool kFlow>
 __global__ void __launch_bounds__(kIluThreads)
     ilu_row_lds_kernel(const int* __restrict__ level_rows, int nrows,
                        const int* __restrict__ rowp,
                        const int* __restrict__ cols,
-                       const int* __restrict__ diag, T* vals, int* pivot) {
+                       const int* __restrict__ diag, T* vals, int* pivot,
+                       Flow f) {
   extern __shared__ __align__(16) unsigned char ilu_dyn[];
   __shared__ int pos[2][kMaxRowBlocks];  // local positions in row i, or -1
   __shared__ T udg[2][36];
   __shared__ T udi[2][6];  // 1 / diagonal of U_kk
   __shared__ T lik[36];
-  const int li = blockIdx.x;
-  if (li >= nrows) return;
-  const int i = level_rows[li];
+  __shared__ int s_row, s_rdy[2];
+  const int i = flow_row<kFlow>(level_rows, nrows, f, &s_row);
+  if (i < 0) return;
   const int tid = threadIdx.x, nth = blockDim.x;
   const int r0 = rowp[i], r1 = rowp[i + 1], nb = r1 - r0, dl = diag[i] - r0;
   T* row = reinterpret_cast<T*>(ilu_dyn);
@@ -280,7 +399,14 @@ __global__ void __launch_bounds__(kIluThreads)
       ubs[2 * t + 1] = rowp[j + 1];
     }
   }
+  if (tid == 0) s_rdy[0] = 0;
+  int par = 0;
   __syncthreads();
+  // dataflow: the values of pivot row scol[kl] are read one (U_kj) or two
+  // (U_kk) pivots ahead, so iteration kl waits for pivot kl + 2
+  auto wait_for = [&](int kl) {
+    if (kFlow) flow_wait(scol, dl, min(kl, dl - 1), s_rdy, par, f);
+  };
   const int ts = tid - 6, ns = nth - 6;  // staging threads (tid >= 6)
   // position of column j among scol[lo..nb), or -1
   auto find = [&](int j, int lo) {
@@ -341,6 +467,7 @@ __global__ void __launch_bounds__(kIluThreads)
     }
   };
   if (dl > 0) {
+    wait_for(1);
     // pivot 0 staged directly by all threads
     const int k = scol[0];
     const int u0 = ubs[0], nu = ubs[1] - u0;
@@ -357,6 +484,7 @@ __global__ void __launch_bounds__(kIluThreads)
   for (int kl = 0; kl < dl; kl++) {
     const int cur = kl & 1;
     const int u0 = ubs[2 * kl], nu = ubs[2 * kl + 1] - u0;
+    if (kl + 1 < dl) wait_for(kl + 2);
     load_u(kl + 1, un);
     if (tid < 6) {
       const int r = tid;
@@ -454,6 +582,7 @@ __global__ void __launch_bounds__(kIluThreads)
   }
   __syncthreads();
   for (int e = tid; e < 36 * nb; e += nth) vals[36 * (size_t)r0 + e] = row[e];
+  if (kFlow) flow_publish(i, f);
 }
 
 // Six lanes per row (lane = scalar row r), ten rows per wave.
@@ -945,10 +1074,13 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
       (st = up(&h->uchunk, uch.data(), uch.size())) ||
       (st = gf::dmalloc(ctx, (void**)&h->pivot, sizeof(int))) ||
       (st = gf::dmalloc(ctx, (void**)&h->counter, sizeof(unsigned long long))) ||
-      (st = gf::dmalloc(ctx, (void**)&h->err, sizeof(int))) ||
+      (st = gf::dmalloc(ctx, (void**)&h->err, 2 * sizeof(int))) ||
+      (st = gf::dmalloc(ctx, (void**)&h->done, sizeof(int) * (size_t)n)) ||
+      (st = gf::check_hip(ctx, hipMemsetAsync(h->done, 0, sizeof(int) * (size_t)n,
+                                              ctx->stream), "memset")) ||
       (st = gf::check_hip(ctx, hipMemsetAsync(h->counter, 0, sizeof(unsigned long long),
                                               ctx->stream), "memset")) ||
-      (st = gf::check_hip(ctx, hipMemsetAsync(h->err, 0, sizeof(int), ctx->stream),
+      (st = gf::check_hip(ctx, hipMemsetAsync(h->err, 0, 2 * sizeof(int), ctx->stream),
                           "memset"))) {
     gf_ilu_destroy(h);
     return st;
@@ -990,6 +1122,7 @@ int gf_ilu_destroy(gf_ilu h) {
   gf::dfree(h->ctx, h->uchunk);
   gf::dfree(h->ctx, h->counter);
   gf::dfree(h->ctx, h->err);
+  gf::dfree(h->ctx, h->done);
   delete h;
   return GF_OK;
 }
@@ -1043,45 +1176,119 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   static std::once_flag once[2];
   const int ti = sizeof(T) == 8 ? 0 : 1;
   std::call_once(once[ti], [&] {
-    hipFuncAttributes fa{};
+    const void* fns[2] = {(const void*)gf::ilu_row_lds_kernel<T, false>,
+                          (const void*)gf::ilu_row_lds_kernel<T, true>};
     hipDeviceProp_t prop{};
-    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess ||
-        hipFuncGetAttributes(&fa, (const void*)gf::ilu_row_lds_kernel<T>) !=
-            hipSuccess)
-      return;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return;
     const size_t total = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
-    if (total <= fa.sharedSizeBytes + 1024) return;
-    const size_t cap = total - fa.sharedSizeBytes;
-    if (hipFuncSetAttribute((const void*)gf::ilu_row_lds_kernel<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)cap) != hipSuccess) {
-      hipGetLastError();
-      if (cap > 64 * 1024 - fa.sharedSizeBytes) return;
+    size_t cap = total;
+    for (const void* fn : fns) {
+      hipFuncAttributes fa{};
+      if (hipFuncGetAttributes(&fa, fn) != hipSuccess) return;
+      if (total <= fa.sharedSizeBytes + 1024) return;
+      cap = std::min(cap, total - fa.sharedSizeBytes);
     }
+    for (const void* fn : fns)
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)cap) != hipSuccess) {
+        hipGetLastError();
+        cap = std::min<size_t>(cap, 48 * 1024);
+      }
     lds_cap[ti] = cap;
   });
   const bool no_lds = getenv("GPUFEM_ILU_GLOBAL") != nullptr;
+  // GPUFEM_ILU_LDS_CAP (bytes) lowers the LDS budget of a row, so that one
+  // factorisation mixes LDS and global-memory rows (a test hook)
+  size_t cap = lds_cap[ti];
+  if (const char* cenv = getenv("GPUFEM_ILU_LDS_CAP"))
+    cap = std::min<size_t>(cap, (size_t)std::max(0L, atol(cenv)));
+  const char* senv = getenv("GPUFEM_ILU_SCHEDULE");
+  const bool by_level = senv && std::string(senv) == "level";
   const int nl = (int)h->lptr.size() - 1;
-  for (int l = 0; l < nl; l++) {
-    const int cnt = h->lptr[l + 1] - h->lptr[l];
-    const size_t dyn =
-        (size_t)h->lmaxlen[l] * (36 * sizeof(T) + 3 * sizeof(int));
-    if (!no_lds && dyn <= lds_cap[ti])
-      hipLaunchKernelGGL(gf::ilu_row_lds_kernel<T>, dim3(cnt),
-                         dim3(gf::kIluThreads), dyn, ctx->stream,
-                         h->lrows + h->lptr[l], cnt, h->rowp, h->cols, h->diag,
-                         vals, h->pivot);
-    else
-      hipLaunchKernelGGL(gf::ilu_row_kernel<T>, dim3(cnt),
-                         dim3(gf::kIluThreads), 0, ctx->stream,
-                         h->lrows + h->lptr[l], cnt, h->rowp, h->cols, h->diag,
-                         vals, h->pivot);
+  auto dyn_of = [&](int l) {
+    return (size_t)h->lmaxlen[l] * (36 * sizeof(T) + 3 * sizeof(int));
+  };
+  if (by_level) {
+    for (int l = 0; l < nl; l++) {
+      const int cnt = h->lptr[l + 1] - h->lptr[l];
+      const size_t dyn = dyn_of(l);
+      if (!no_lds && dyn <= cap)
+        hipLaunchKernelGGL((gf::ilu_row_lds_kernel<T, false>), dim3(cnt),
+                           dim3(gf::kIluThreads), dyn, ctx->stream,
+                           h->lrows + h->lptr[l], cnt, h->rowp, h->cols,
+                           h->diag, vals, h->pivot, gf::Flow{});
+      else
+        hipLaunchKernelGGL((gf::ilu_row_kernel<T, false>), dim3(cnt),
+                           dim3(gf::kIluThreads), 0, ctx->stream,
+                           h->lrows + h->lptr[l], cnt, h->rowp, h->cols,
+                           h->diag, vals, h->pivot, gf::Flow{});
+    }
+  } else {
+    // Dataflow: one launch per phase.  Levels before the first level with a
+    // row too long for LDS run the LDS kernel, in phases whose LDS need
+    // stays within twice that of their first level or 40 KB (so the wide
+    // early levels keep several workgroups per CU); every level from there
+    // on runs the global-memory kernel in one phase (row lengths of a
+    // complete factorisation oscillate around the LDS limit, and a launch
+    // boundary is a barrier for every row).  Launch boundaries order the
+    // phases.
+    if (++h->epoch <= 0) {
+      GF_HIP(ctx, hipMemsetAsync(h->done, 0, sizeof(int) * (size_t)h->n,
+                                 ctx->stream));
+      h->epoch = 1;
+    }
+    int first_global = 0;
+    while (first_global < nl && !no_lds && dyn_of(first_global) <= cap)
+      first_global++;
+    int l0 = 0;
+    while (l0 < nl) {
+      const bool lds = l0 < first_global;
+      const size_t d0 = dyn_of(l0);
+      const size_t lim = std::max<size_t>(2 * d0, 40 * 1024);
+      size_t dyn = d0;
+      int l1 = l0 + 1;
+      if (lds) {
+        for (; l1 < first_global && dyn_of(l1) <= lim; l1++)
+          dyn = std::max(dyn, dyn_of(l1));
+      } else {
+        l1 = nl;
+      }
+      const int r0 = h->lptr[l0], cnt = h->lptr[l1] - r0;
+      gf::Flow f;
+      f.counter = h->counter;
+      f.base = h->base;
+      f.done = h->done;
+      f.epoch = h->epoch;
+      f.err = h->err + 1;
+      if (lds)
+        hipLaunchKernelGGL((gf::ilu_row_lds_kernel<T, true>), dim3(cnt),
+                           dim3(gf::kIluThreads), dyn, ctx->stream,
+                           h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals,
+                           h->pivot, f);
+      else
+        hipLaunchKernelGGL((gf::ilu_row_kernel<T, true>), dim3(cnt),
+                           dim3(gf::kIluThreads), 0, ctx->stream,
+                           h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals,
+                           h->pivot, f);
+      h->base += (unsigned long long)cnt;
+      l0 = l1;
+    }
   }
   GF_TRY(gf::check_launch(ctx, "ilu_row_kernel"));
-  int piv = big;
+  int piv = big, werr = 0;
   GF_HIP(ctx, hipMemcpyAsync(&piv, h->pivot, sizeof(int),
                              hipMemcpyDeviceToHost, ctx->stream));
+  GF_HIP(ctx, hipMemcpyAsync(&werr, h->err + 1, sizeof(int),
+                             hipMemcpyDeviceToHost, ctx->stream));
   GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (werr) {
+    const int zero = 0;
+    GF_HIP(ctx, hipMemcpyAsync(h->err + 1, &zero, sizeof(int),
+                               hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return gf::set_error(ctx, GF_ERR_TIMEOUT,
+                         "ILU factorisation: a dependency wait expired");
+  }
   if (zero_pivot) *zero_pivot = piv == big ? -1 : piv;
   if (piv != big)
     return gf::set_error(ctx, GF_ERR_ZERO_PIVOT,

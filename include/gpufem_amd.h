@@ -181,7 +181,11 @@ const int* gf_ilu_device_cols(gf_ilu ilu);
  * check (synchronises the context stream) */
 int gf_ilu_check(gf_ilu ilu);
 /* in place on device values laid out on the ILU pattern; on
- * GF_ERR_ZERO_PIVOT *zero_pivot holds the first singular block row */
+ * GF_ERR_ZERO_PIVOT *zero_pivot holds the first singular block row.
+ * Rows are factored on a dataflow schedule (each row waits for its own pivot
+ * rows; GPUFEM_ILU_SCHEDULE=level runs one launch per level instead, with
+ * bitwise-identical factors); GF_ERR_TIMEOUT if a bounded wait expired.
+ * Synchronises the context stream. */
 int gf_bsr_ilu_factor_f64(gf_ilu ilu, double* d_vals, int* zero_pivot);
 int gf_bsr_ilu_factor_f32(gf_ilu ilu, float* d_vals, int* zero_pivot);
 /* x = U^-1 L^-1 b with the factored values (b and x may not alias) */

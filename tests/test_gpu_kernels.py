@@ -264,6 +264,82 @@ def test_ilu_lds_rows_match_global_rows(gpu_ctx, reorder, k, monkeypatch):
     ilu.close()
 
 
+ILU_SCHEDULE_CASES = [
+    # (name, mesh, ordering, level of fill, LDS budget or None, global rows)
+    ("plate24_amd_ilu12", lambda: M.plate(24, 24, bc="ss", load="sine"),
+     "amd", 12, None, False),
+    ("plate24_rcm_ilu3_global", lambda: M.plate(24, 24, bc="ss", load="sine"),
+     "rcm", 3, None, True),
+    # complete factorisation (rows up to 250 blocks): several LDS phases
+    ("wingbox2176_amd_lu", lambda: M.wingbox(nc=24, nh=4, nsb=4, nribs=8,
+                                             seed=5),
+     "amd", 1 << 29, None, False),
+    # rows over 160 blocks in global memory: LDS and global phases mixed
+    ("wingbox2176_amd_lu_mixed", lambda: M.wingbox(nc=24, nh=4, nsb=4,
+                                                   nribs=8, seed=5),
+     "amd", 1 << 29, 48 * 1024, False),
+]
+
+
+@pytest.mark.parametrize("case", ILU_SCHEDULE_CASES, ids=lambda c: c[0])
+def test_ilu_dataflow_matches_level_schedule(gpu_ctx, case, monkeypatch):
+    """The dataflow factorisation (rows taken in level order, each waiting
+    only for its pivot rows) and the level-by-level factorisation are two
+    schedules of the same row arithmetic: the factors agree bit for bit,
+    FP64 and FP32, over repeated factorisations (flags reused by epoch)."""
+    name, make, reorder, k, cap, glob = case
+    say(f"[gpu] ilu dataflow vs level schedule {name}")
+    m = make()
+    s = oracle_problem(m, reorder, k)
+    if glob:
+        monkeypatch.setenv("GPUFEM_ILU_GLOBAL", "1")
+    if cap is not None:
+        monkeypatch.setenv("GPUFEM_ILU_LDS_CAP", str(cap))
+    ilu = G.Ilu(gpu_ctx, s["mrp"], s["mcp"])
+    for dt in (np.float64, np.float32):
+        monkeypatch.setenv("GPUFEM_ILU_SCHEDULE", "level")
+        d_ref = dev(s["mv"].astype(dt))
+        ilu.factor(d_ref)
+        gpu_ctx.sync()
+        ref = host(d_ref)
+        monkeypatch.delenv("GPUFEM_ILU_SCHEDULE")
+        for rep in range(2):
+            d_a = dev(s["mv"].astype(dt))
+            ilu.factor(d_a)
+            gpu_ctx.sync()
+            a = host(d_a)
+            assert np.array_equal(a.view(np.uint8), ref.view(np.uint8)), \
+                f"{name} {np.dtype(dt).name} rep {rep}"
+    ilu.close()
+
+
+def test_ilu_dataflow_complete_factors(gpu_ctx):
+    """Complete factorisation (every fill kept) on the dataflow schedule:
+    the factors match the oracle's block LU row by row, and L U reproduces
+    the matrix everywhere (an exact LU, no dropped fill)."""
+    say("[gpu] ilu dataflow complete LU wingbox 254 nodes")
+    from test_oracle_numeric import bsr_to_dense
+
+    m = M.wingbox(nc=10, nh=3, nsb=2, nribs=4, seed=5)
+    s = oracle_problem(m, "amd", 1 << 29)
+    lu_o, st = oracle.bsr_ilu(s["mrp"], s["mcp"], s["mv"])
+    assert st == 0
+    ilu = G.Ilu(gpu_ctx, s["mrp"], s["mcp"])
+    d_lu = dev(s["mv"].copy())
+    ilu.factor(d_lu)
+    gpu_ctx.sync()
+    lu = host(d_lu)
+    ilu.close()
+    rp = s["mrp"]
+    for i in range(len(rp) - 1):
+        a, b = lu[36 * rp[i]:36 * rp[i + 1]], lu_o[36 * rp[i]:36 * rp[i + 1]]
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max()
+    Dm = bsr_to_dense(s["mrp"], s["mcp"], lu)
+    A = bsr_to_dense(s["krp"], s["kcp"], s["kv"])
+    LU = (np.tril(Dm, -1) + np.eye(len(Dm))) @ np.triu(Dm)
+    assert np.abs(LU - A).max() <= 1e-10 * np.abs(A).max()
+
+
 def test_zero_pivot_reported(gpu_ctx):
     say("[gpu] zero pivot detection")
     m = M.plate(3, 3)
