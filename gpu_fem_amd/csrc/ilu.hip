@@ -813,7 +813,13 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
 #pragma unroll
       for (int o = G; o < 64; o <<= 1)
         npass = max(npass, __shfl_xor(npass, o, 64));
-      for (int ps = 0; ps < npass; ps++) {
+      // the row's latest dependency is usually its nearest row: the last
+      // block of an L row, the first of a U row.  U rows take their passes
+      // last to first, so that the pass that waits longest comes last and
+      // the others overlap with the wait (rows of over kGather blocks: the
+      // chains of a complete factorisation)
+      for (int pi = 0; pi < npass; pi++) {
+        const int ps = kLower ? pi : npass - 1 - pi;
         const int g0b = k0 + ps * kGather;
         const int ng = max(0, min(kGather, k1 - g0b));
         T a[kSweepIter][6];
