@@ -122,12 +122,23 @@ _SIGS = {
     "gf_server_stats": [_P, _P],
     "gf_server_drain": [_P],
     "gf_server_destroy": [_P],
+    "gf_bdf_read": [ctypes.c_char_p, ctypes.POINTER(_P)],
+    "gf_bdf_error": [],
+    "gf_bdf_destroy": [_P],
+    "gf_bdf_sizes": [_P, _PI, _PI, _PI, _PI, _PI, _PI],
+    "gf_bdf_connectivity": [_P, _P, _P, _P, _P],
+    "gf_bdf_bcs": [_P, _P, _P, _P, _P],
+    "gf_bdf_assembler_data": [_P, _I, _P, _P, _P],
+    "gf_bdf_component_descript": [_P, _I],
+    "gf_bdf_element_descript": [_P, _I],
 }
 _RESTYPES = {"gf_context_last_error": ctypes.c_char_p,
              "gf_context_stream": _P, "gf_free": None,
              "gf_options_default": None, "gf_ilu_device_rowp": _P,
              "gf_problem_ilu": _P,
-             "gf_ilu_device_cols": _P}
+             "gf_ilu_device_cols": _P, "gf_bdf_error": ctypes.c_char_p,
+             "gf_bdf_component_descript": ctypes.c_char_p,
+             "gf_bdf_element_descript": ctypes.c_char_p}
 
 
 def exported_symbols():
@@ -151,6 +162,11 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: run __graft_entry__.build() or "
                 "python gpu_fem_amd/build.py (there is no CPU fallback)")
+        # torch ships its own HIP and HSA runtimes; loading this library
+        # first would map the system ones as well, and two HSA runtimes in
+        # one process conflict over the device.  With torch loaded first the
+        # library binds to torch's runtime (same sonames).
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, args in _SIGS.items():
             f = getattr(L, name)
@@ -255,6 +271,78 @@ def transpose_map(rowp, cols):
 def bandwidth(rowp, cols):
     rowp, cols = _i32(rowp), _i32(cols)
     return lib().gf_bandwidth(len(rowp) - 1, ptr(rowp), ptr(cols))
+
+
+class MeshLoader:
+    """NASTRAN bulk-data reader with the reference's call surface
+    (TACSMeshLoader, include/mesh/TACSMeshLoader.h:145-227): scan_bdf_file,
+    then the node / element / BC arrays.  Host only."""
+
+    def __init__(self):
+        self.h = None
+
+    def scan_bdf_file(self, path):
+        """scanBDFFile (:763); raises GfError on an unreadable or bad file."""
+        self.close()
+        h = _P()
+        code = lib().gf_bdf_read(os.fsencode(path), ctypes.byref(h))
+        if code != 0:
+            raise GfError(code, lib().gf_bdf_error().decode())
+        self.h = h
+        c = [ctypes.c_int() for _ in range(6)]
+        check(lib().gf_bdf_sizes(self.h, *[ctypes.byref(v) for v in c]))
+        (self.num_nodes, self.num_elements, self._conn_size, self._nbcs,
+         self._nbc_vars, self.num_components) = [v.value for v in c]
+        return self
+
+    def connectivity(self):
+        """getConnectivity (:184-187): xpts [nnodes, 3], elem_ptr, conn,
+        elem_comp (property id - 1)."""
+        x = np.zeros(3 * self.num_nodes)
+        ep = np.zeros(self.num_elements + 1, np.int32)
+        cn = np.zeros(self._conn_size, np.int32)
+        cp = np.zeros(self.num_elements, np.int32)
+        check(lib().gf_bdf_connectivity(self.h, ptr(x), ptr(ep), ptr(cn),
+                                        ptr(cp)))
+        return x.reshape(-1, 3), ep, cn, cp
+
+    def bcs(self):
+        """getBCs (:188-189): bc_nodes, bc_ptr, bc_vars, bc_vals."""
+        bn = np.zeros(self._nbcs, np.int32)
+        bp = np.zeros(self._nbcs + 1, np.int32)
+        bv = np.zeros(self._nbc_vars, np.int32)
+        bx = np.zeros(self._nbc_vars)
+        check(lib().gf_bdf_bcs(self.h, ptr(bn), ptr(bp), ptr(bv), ptr(bx)))
+        return bn, bp, bv, bx
+
+    def assembler_data(self, vars_per_node=6):
+        """getAssemblerCreatorData (:193-224): elem_conn, global BC DOFs
+        (vars_per_node * node + var), xpts."""
+        cn = np.zeros(self._conn_size, np.int32)
+        b = np.zeros(self._nbc_vars, np.int32)
+        x = np.zeros(3 * self.num_nodes)
+        check(lib().gf_bdf_assembler_data(self.h, vars_per_node, ptr(cn),
+                                          ptr(b), ptr(x)))
+        return cn, b, x.reshape(-1, 3)
+
+    def component_descript(self, comp):
+        r = lib().gf_bdf_component_descript(self.h, comp)
+        return None if r is None else r.decode()
+
+    def element_descript(self, comp):
+        r = lib().gf_bdf_element_descript(self.h, comp)
+        return None if r is None else r.decode()
+
+    def close(self):
+        if self.h:
+            lib().gf_bdf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # --------------------------------------------------------- device objects

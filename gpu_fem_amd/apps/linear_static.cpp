@@ -5,9 +5,12 @@
 // bsr_post_solve).  Differences from that program: GMRES is the library's
 // restarted right-preconditioned solver with a relative tolerance, and the
 // connectivity order is an option (--ccw reproduces the reference generator).
+// --bdf <file> reads the mesh from a NASTRAN bulk-data file instead, with one
+// material (E = 70 GPa, nu = 0.3, t = 5 mm) and fz = 10 on every node, the
+// set-up of examples/uCRM/linear_static.cu:13-80 (nxe is then ignored).
 //
 // usage: linear_static <nxe> <levFill> <rcm 0|1> <restart> <rtol> <out.bin>
-//        [--ccw] [--direct]
+//        [--ccw] [--direct] [--bdf file]
 // prints one JSON line; writes the unpermuted solution (6*nnodes doubles).
 #include <cstdio>
 #include <cstring>
@@ -20,7 +23,7 @@ using namespace gpufem;
 int main(int argc, char** argv) {
   if (argc < 7) {
     std::fprintf(stderr, "usage: %s nxe levFill rcm restart rtol out.bin "
-                         "[--ccw] [--direct]\n", argv[0]);
+                         "[--ccw] [--direct] [--bdf file]\n", argv[0]);
     return 2;
   }
   const int nxe = std::atoi(argv[1]), nye = nxe;
@@ -30,9 +33,11 @@ int main(int argc, char** argv) {
   const double tolerance = std::atof(argv[5]);
   const char* out = argv[6];
   bool ccw = false, direct = false;
+  const char* bdf = nullptr;
   for (int i = 7; i < argc; i++) {
     if (!std::strcmp(argv[i], "--ccw")) ccw = true;
     if (!std::strcmp(argv[i], "--direct")) direct = true;
+    if (!std::strcmp(argv[i], "--bdf") && i + 1 < argc) bdf = argv[++i];
   }
 
   using T = double;
@@ -46,11 +51,24 @@ int main(int argc, char** argv) {
   using Assembler = ElementAssembler<T, ElemGroup, VecType, BsrMat>;
 
   const double Lx = 2.0, Ly = 1.0, E = 70e9, nu = 0.3, thick = 0.005;
-  auto assembler =
-      createPlateAssembler<Assembler>(nxe, nye, Lx, Ly, E, nu, thick, ccw);
-  auto loads_h = getPlateLoads<T, Physics>(nxe, nye, Lx, Ly, 1.0);
-
+  TACSMeshLoader<T> mesh_loader;
+  if (bdf && mesh_loader.scanBDFFile(bdf)) {
+    std::fprintf(stderr, "linear_static: %s\n", mesh_loader.error().c_str());
+    return 1;
+  }
   try {
+    auto assembler =
+        bdf ? Assembler::createFromBDF(mesh_loader, Data(E, nu, thick))
+            : createPlateAssembler<Assembler>(nxe, nye, Lx, Ly, E, nu, thick,
+                                              ccw);
+    std::vector<T> loads_h;
+    if (bdf) {
+      loads_h.assign(assembler.get_num_vars(), T(0));
+      for (int i = 0; i < assembler.get_num_nodes(); i++) loads_h[6 * i + 2] = 10.0;
+    } else {
+      auto pl = getPlateLoads<T, Physics>(nxe, nye, Lx, Ly, 1.0);
+      loads_h.assign(pl.begin(), pl.end());
+    }
     if (direct) {
       // direct path: AMD + complete fill (plate_demo/linear_static.cu:29-67)
       assembler.symbolic_factorization(10.0, false);

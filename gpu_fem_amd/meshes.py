@@ -17,6 +17,7 @@ unpermuted node numbering; loads have ``6*nnodes`` entries.
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -329,6 +330,80 @@ def wingbox(nc=120, nh=16, nsb=30, nribs=50, span=30.0, c_root=6.0,
                 meta=dict(nc=nc, nh=nh, nsb=nsb, nribs=nribs, span=span,
                           c_root=c_root, c_tip=c_tip, h_frac=h_frac, seed=seed,
                           panels=5 * nribs - 4))
+
+
+def from_bdf(path, E=70e9, nu=0.3, thick=0.005, loads=None, name=None):
+    """Shell mesh from a NASTRAN bulk-data file, one material and thickness
+    for every element (ElementAssembler::createFromBDF with one Data,
+    include/assembler.h:92-115; the uCRM drivers use E = 70 GPa, nu = 0.3,
+    t = 5 mm, examples/uCRM/linear_static.cu:31-34).  Every element must be
+    a 4-node quad (CQUAD4 / CQUADR); BCs are the SPC DOFs as 6*node + var.
+    ``loads``: None (zero), an array of 6*nnodes, or a callable
+    f(conn, xpts) returning one."""
+    from . import MeshLoader
+
+    ml = MeshLoader().scan_bdf_file(path)
+    try:
+        xpts, eptr, conn, _ = ml.connectivity()
+        if ml.num_elements and not np.all(np.diff(eptr) == 4):
+            bad = int(np.argmax(np.diff(eptr) != 4))
+            raise ValueError(f"{path}: element {bad} (in id order) is not a "
+                             f"4-node quad ({ml.element_descript(0)} ...)")
+        _, _, bv, _ = ml.bcs()
+        if np.any(bv >= 6):
+            raise ValueError(f"{path}: SPC component digits 7-8 do not exist "
+                             "for a 6-DOF shell node")
+        _, bcs, _ = ml.assembler_data(6)
+    finally:
+        ml.close()
+    conn = conn.reshape(-1, 4).astype(np.int32)
+    nn = xpts.shape[0]
+    if loads is None:
+        f = np.zeros(6 * nn)
+    elif callable(loads):
+        f = np.asarray(loads(conn, xpts), dtype=np.float64)
+    else:
+        f = np.asarray(loads, dtype=np.float64).copy()
+    if f.shape != (6 * nn,):
+        raise ValueError(f"loads must have {6 * nn} entries")
+    return Mesh(conn=conn, xpts=np.ascontiguousarray(xpts),
+                data=_data(conn.shape[0], E, nu, thick), bcs=bcs.astype(np.int32),
+                loads=f, name=name or os.path.splitext(os.path.basename(path))[0])
+
+
+def to_bdf(mesh, path, large_field=False):
+    """Write a mesh's nodes, CQUAD4 elements (property 1) and SPCs as NASTRAN
+    bulk data, small (8-column) or large (16-column) field: from_bdf reads
+    back the same connectivity and BCs, and the coordinates rounded to 10
+    significant digits (large field) or 5 (small field)."""
+    lines = ["BEGIN BULK"]
+    for i, (x, y, z) in enumerate(mesh.xpts):
+        if large_field:
+            lines.append(f"GRID*   {i + 1:<16d}{0:<16d}{x:<16.9E}{y:<16.9E}*")
+            lines.append(f"*       {z:<16.9E}")
+        else:
+            lines.append(f"GRID    {i + 1:<8d}{0:<8d}{_f8(x)}{_f8(y)}{_f8(z)}")
+    for e, c in enumerate(mesh.conn):
+        g = [c[0] + 1, c[1] + 1, c[3] + 1, c[2] + 1]  # around the element
+        lines.append(f"CQUAD4  {e + 1:<8d}{1:<8d}" + "".join(f"{v:<8d}" for v in g))
+    by_node = {}
+    for d in np.asarray(mesh.bcs):
+        by_node.setdefault(int(d) // 6, []).append(int(d) % 6 + 1)
+    for nd, comps in sorted(by_node.items()):
+        cs = "".join(str(v) for v in sorted(set(comps)))
+        lines.append(f"SPC     {1:<8d}{nd + 1:<8d}{cs:<8s}{0.0:<8.1f}")
+    lines.append("ENDDATA")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def _f8(v):
+    """A real in 8 columns (NASTRAN small field)."""
+    for p in range(7, -1, -1):
+        s = f"{v:.{p}E}".replace("E+0", "+").replace("E-0", "-").replace("E+", "+").replace("E-", "-")
+        if len(s) <= 8:
+            return f"{s:<8s}"
+    raise ValueError(f"{v} does not fit 8 columns")
 
 
 def served_requests(n, seed=0, scale=1.0):

@@ -9,7 +9,8 @@
 //   BsrMat                               include/linalg/bsr_mat.h:10-356
 //   ILUk_Preconditioner                  include/linalg/precond.h:5-55
 //   bsr_pre_solve / bsr_post_solve       include/linalg/solvers/solve_utils.h
-//   ElementAssembler                     include/assembler.h:14-321
+//   ElementAssembler (+ createFromBDF)   include/assembler.h:14-321
+//   TACSMeshLoader                       include/mesh/TACSMeshLoader.h:138-227
 //   shell element type tags              include/shell/{quadrature,director,
 //                                        basis,data,physics,elem_group}.h
 //   TacsComputeRCMOrder, computeILUk,    include/linalg/reordering_utils.h:272,
@@ -597,6 +598,73 @@ struct ShellElementGroup {
   using Phys = Phys_;
 };
 
+// ------------------------------------------------------- TACSMeshLoader
+// NASTRAN bulk-data reader (include/mesh/TACSMeshLoader.h:145-227) on
+// gf_bdf_read.  scanBDFFile returns 0 or 1 as the reference does (the reason
+// is in error()); getAssemblerCreatorData hands out new[] arrays the caller
+// deletes, as the reference does.
+template <typename T>
+class TACSMeshLoader {
+ public:
+  TACSMeshLoader() = default;
+  TACSMeshLoader(const TACSMeshLoader&) = delete;
+  TACSMeshLoader& operator=(const TACSMeshLoader&) = delete;
+  ~TACSMeshLoader() { gf_bdf_destroy(h_); }
+
+  int scanBDFFile(const char* file_name) {
+    gf_bdf_destroy(h_);
+    h_ = nullptr;
+    if (gf_bdf_read(file_name, &h_) != GF_OK) {
+      error_ = gf_bdf_error();
+      return 1;
+    }
+    gf_bdf_sizes(h_, &num_nodes_, &num_elements_, &conn_size_, nullptr,
+                 &num_bc_vars_, &num_components_);
+    return 0;
+  }
+  const std::string& error() const { return error_; }
+  // extensions: total element nodes, and whether every SPC component digit
+  // names one of nv DOFs (digits 7-8 would spill into the next node)
+  int getElemConnSize() { return conn_size_; }
+  bool bcVarsWithin(int nv) {
+    std::vector<int> v(num_bc_vars_ > 0 ? num_bc_vars_ : 1);
+    gf_bdf_bcs(h_, nullptr, nullptr, v.data(), nullptr);
+    for (int j = 0; j < num_bc_vars_; j++)
+      if (v[j] >= nv) return false;
+    return true;
+  }
+  int getNumNodes() { return num_nodes_; }
+  int getNumElements() { return num_elements_; }
+  int getNumComponents() { return num_components_; }
+  const char* getComponentDescript(int comp) {
+    return gf_bdf_component_descript(h_, comp);
+  }
+  const char* getElementDescript(int comp) {
+    return gf_bdf_element_descript(h_, comp);
+  }
+  void getAssemblerCreatorData(const int vars_per_node, int& my_num_nodes,
+                               int& my_num_elements, int& my_num_bcs,
+                               int*& my_elem_conn, int*& my_bcs,
+                               T*& my_xpts) {
+    if (!h_) throw Error(GF_ERR_INVALID, "getAssemblerCreatorData: no mesh");
+    my_num_nodes = num_nodes_;
+    my_num_elements = num_elements_;
+    my_num_bcs = num_bc_vars_;
+    my_elem_conn = new int[conn_size_ > 0 ? conn_size_ : 1];
+    my_bcs = new int[num_bc_vars_ > 0 ? num_bc_vars_ : 1];
+    std::vector<double> x(3 * (size_t)num_nodes_);
+    gf_bdf_assembler_data(h_, vars_per_node, my_elem_conn, my_bcs, x.data());
+    my_xpts = new T[3 * (size_t)num_nodes_ + 1];
+    for (size_t i = 0; i < x.size(); i++) my_xpts[i] = T(x[i]);
+  }
+
+ private:
+  gf_bdf h_ = nullptr;
+  std::string error_;
+  int num_nodes_ = 0, num_elements_ = 0, conn_size_ = 0, num_bc_vars_ = 0,
+      num_components_ = 0;
+};
+
 // --------------------------------------------------------- ElementAssembler
 template <typename T_, typename ElemGroup, template <typename> class Vec,
           template <typename> class Mat_>
@@ -634,6 +702,31 @@ class ElementAssembler {
       d[4 * e + 3] = physData[e].tOffset;
     }
     data_ = d.createDeviceVec();
+  }
+
+  // include/assembler.h:92-115: one Data for every element; every element
+  // must be a 4-node quad (CQUAD4 / CQUADR)
+  static ElementAssembler createFromBDF(TACSMeshLoader<T>& mesh_loader,
+                                        Data single_data) {
+    int num_nodes = 0, num_elements = 0, num_bcs = 0;
+    int *elem_conn = nullptr, *bcs = nullptr;
+    T* xpts = nullptr;
+    mesh_loader.getAssemblerCreatorData(vars_per_node, num_nodes, num_elements,
+                                        num_bcs, elem_conn, bcs, xpts);
+    std::unique_ptr<int[]> c(elem_conn), b(bcs);
+    std::unique_ptr<T[]> x(xpts);
+    if (mesh_loader.getElemConnSize() != 4 * num_elements)
+      throw Error(GF_ERR_INVALID, "createFromBDF: every element must be a "
+                                  "4-node quad (CQUAD4 / CQUADR)");
+    if (!mesh_loader.bcVarsWithin(vars_per_node))
+      throw Error(GF_ERR_INVALID, "createFromBDF: SPC component digit beyond "
+                                  "the node's DOFs");
+    HostVec<int32_t> conn_vec(4 * num_elements, elem_conn);
+    HostVec<int> bcs_vec(num_bcs, bcs);
+    HostVec<T> xpts_vec(3 * num_nodes, xpts);
+    HostVec<Data> phys(num_elements, single_data);
+    return ElementAssembler(num_nodes, num_nodes, num_elements, conn_vec,
+                            conn_vec, xpts_vec, bcs_vec, phys);
   }
 
   BsrData getBsrData() { return bsr_data; }

@@ -45,6 +45,7 @@ typedef struct gf_context_s* gf_context;
 typedef struct gf_ilu_s* gf_ilu;
 typedef struct gf_problem_s* gf_problem;
 typedef struct gf_server_s* gf_server;
+typedef struct gf_bdf_s* gf_bdf;
 
 /* ---- context (replaces the implicit default stream + exit() on error of
  *      include/cuda_utils.h:33-83) ---- */
@@ -275,6 +276,37 @@ int gf_server_poll(gf_server s, long long* id, double* x, size_t x_len,
 int gf_server_stats(gf_server s, double* stats);
 int gf_server_drain(gf_server s);
 int gf_server_destroy(gf_server s);
+
+/* ---- NASTRAN bulk-data mesh reader (TACSMeshLoader,
+ *      include/mesh/TACSMeshLoader.h:145-227, 763-1290; host only).
+ *      GRID / GRID* / free-field GRID, SPC / SPC* / SPC1, and the element
+ *      cards CBAR CQUADR CQUAD4 CQUAD8 CQUAD9 CQUAD CHEXA27 CHEXA CTRIA3
+ *      CTETRA; nodes and elements ordered by file id, CQUAD4/CQUADR corners
+ *      stored in tensor-product order (1 2 4 3). ---- */
+/* scanBDFFile (:763); GF_ERR_INVALID with gf_bdf_error() on a bad file */
+int gf_bdf_read(const char* path, gf_bdf* out);
+/* the reason the last gf_bdf_read on this thread failed */
+const char* gf_bdf_error(void);
+int gf_bdf_destroy(gf_bdf b);
+/* getNumNodes / getNumElements / getNumComponents (:158-160); conn_size =
+ * total element nodes, nbcs = SPC entries, nbc_vars = constrained DOFs */
+int gf_bdf_sizes(gf_bdf b, int* nnodes, int* nelems, int* conn_size,
+                 int* nbcs, int* nbc_vars, int* ncomponents);
+/* getConnectivity (:184-187): xpts[3*nnodes], elem_ptr[nelems+1],
+ * conn[conn_size], elem_comp[nelems] (property id - 1); any may be NULL */
+int gf_bdf_connectivity(gf_bdf b, double* xpts, int* elem_ptr, int* conn,
+                        int* elem_comp);
+/* getBCs (:188-189): bc_nodes[nbcs], bc_ptr[nbcs+1], bc_vars[nbc_vars]
+ * (component digit - 1), bc_vals[nbc_vars] */
+int gf_bdf_bcs(gf_bdf b, int* bc_nodes, int* bc_ptr, int* bc_vars,
+               double* bc_vals);
+/* getAssemblerCreatorData (:193-224): elem_conn[conn_size], bcs[nbc_vars]
+ * = vars_per_node * node + var, xpts[3*nnodes] */
+int gf_bdf_assembler_data(gf_bdf b, int vars_per_node, int* elem_conn,
+                          int* bcs, double* xpts);
+/* getComponentDescript / getElementDescript (:150-151); NULL if out of range */
+const char* gf_bdf_component_descript(gf_bdf b, int comp);
+const char* gf_bdf_element_descript(gf_bdf b, int comp);
 
 #ifdef __cplusplus
 }

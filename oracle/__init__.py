@@ -7,8 +7,10 @@ the HIP library under ``gpu_fem_amd`` neither links nor calls it.
 ``liboracle.so`` restates the reference host path in plain C++ (see the
 citations at the top of ``element.cpp``, ``symbolic.cpp`` and
 ``numeric.cpp``).  ``_ref/libgfref.so`` is the reference's own
-self-contained symbolic code compiled from its sources (``build_ref.sh``);
-it exists only where the reference tree was available at build time.
+self-contained symbolic code compiled from its sources (``build_ref.sh``),
+``_ref/libgfref_bdf.so`` its NASTRAN reader (``ref_bdf.cpp`` around
+include/mesh/TACSMeshLoader.h); they exist only where the reference tree was
+available at build time.
 
 Every oracle function is pinned by a test in ``tests/test_oracle_*.py``
 against the reference's fixtures, its compiled functions, or mathematical
@@ -384,3 +386,46 @@ def ref_node_graph(conn, nnodes, nodes_per_elem=4):
                                nodes_per_elem, ctypes.byref(rp),
                                ctypes.byref(cp))
     return _take(rp, nnodes + 1, L.gfref_free), _take(cp, nnz, L.gfref_free)
+
+
+_REF_BDF = None
+
+
+def ref_bdf_lib():
+    """The reference's NASTRAN reader (``_ref/libgfref_bdf.so``, built from
+    include/mesh/TACSMeshLoader.h by ``build_ref.sh``), or None."""
+    global _REF_BDF
+    if _REF_BDF is None:
+        path = os.path.join(_HERE, "_ref", "libgfref_bdf.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path)
+        L.gfref_bdf_scan.restype = ctypes.c_void_p
+        L.gfref_bdf_scan.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.gfref_bdf_sizes.argtypes = [ctypes.c_void_p, _i32p]
+        L.gfref_bdf_get.argtypes = [ctypes.c_void_p, _i32p, ctypes.c_int,
+                                    _i32p, _f64p]
+        L.gfref_bdf_free.argtypes = [ctypes.c_void_p]
+        _REF_BDF = L
+    return _REF_BDF
+
+
+def ref_bdf(path, vars_per_node=6, nodes_per_elem=4):
+    """Reference reader on a file: None if it reports failure, else
+    (conn [nelems, nodes_per_elem], global BC DOFs, xpts [nnodes, 3]) from
+    TACSMeshLoader::getAssemblerCreatorData."""
+    L = ref_bdf_lib()
+    h = L.gfref_bdf_scan(os.fsencode(path), vars_per_node)
+    try:
+        sz = np.zeros(4, np.int32)
+        L.gfref_bdf_sizes(h, sz)
+        fail, nn, ne, nb = (int(v) for v in sz)
+        if fail:
+            return None
+        conn = np.zeros(nodes_per_elem * ne, np.int32)
+        bcs = np.zeros(nb, np.int32)
+        x = np.zeros(3 * nn)
+        L.gfref_bdf_get(h, conn, len(conn), bcs, x)
+        return conn.reshape(ne, nodes_per_elem), bcs, x.reshape(nn, 3)
+    finally:
+        L.gfref_bdf_free(h)

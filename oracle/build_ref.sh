@@ -1,6 +1,8 @@
 #!/bin/sh
 # Compile the reference's own self-contained symbolic functions into
-# oracle/_ref/libgfref.so so the oracle can be checked against them.
+# oracle/_ref/libgfref.so, and its NASTRAN reader into
+# oracle/_ref/libgfref_bdf.so, so the oracle and the product can be checked
+# against them.
 #
 #   include/linalg/reordering_utils.h   compiled as is (TacsComputeRCMOrder)
 #   include/linalg/bsr_utils.h          only its free functions, taken by line
@@ -98,3 +100,12 @@ EOC
 mkdir -p "$OUT"
 ${CXX:-g++} -O2 -std=c++17 -fPIC -shared -w -I"$TMP" -o "$OUT/libgfref.so" "$TMP/gfref.cpp"
 echo "build_ref: built $OUT/libgfref.so"
+
+# the NASTRAN reader: TACSMeshLoader.h is self-contained (C library headers
+# only) and is compiled where it lies, through the wrapper ref_bdf.cpp
+L="$REF/include/mesh/TACSMeshLoader.h"
+if [ -f "$L" ]; then
+  ${CXX:-g++} -O2 -std=c++17 -fPIC -shared -w -I"$REF/include" \
+    -o "$OUT/libgfref_bdf.so" "$(dirname "$0")/ref_bdf.cpp"
+  echo "build_ref: built $OUT/libgfref_bdf.so"
+fi

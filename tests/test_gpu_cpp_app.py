@@ -68,3 +68,25 @@ def test_direct_path_matches_dense():
     b = oracle.apply_bcs_vec(m.bcs, m.loads)
     xd = oracle.dense_solve(A, b)
     assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+def test_bdf_mesh_direct_path_matches_dense():
+    """The uCRM driver's set-up (examples/uCRM/linear_static.cu:13-80): a
+    NASTRAN bulk-data mesh through TACSMeshLoader and createFromBDF, one
+    material, fz = 10 on every node, direct solve; checked against a dense
+    solve of the same mesh read by the Python reader."""
+    print("[gpu] linear_static app --direct --bdf wingbox 254 nodes", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "wb.bdf")
+        M.to_bdf(M.wingbox(nc=10, nh=3, nsb=2, nribs=4, seed=5), p,
+                 large_field=True)
+        info, x = run_app(0, 0, 0, 10, 1e-10, "--direct", "--bdf", p)
+        m = M.from_bdf(p, E=70e9, nu=0.3, thick=0.005)
+    assert info["path"] == "direct" and info["nnodes"] == m.nnodes
+    f = np.zeros(6 * m.nnodes)
+    f[2::6] = 10.0
+    s = oracle_problem(m, "natural", 0)
+    from test_oracle_numeric import bsr_to_dense
+    A = bsr_to_dense(s["krp"], s["kcp"], s["kv"])
+    xd = oracle.dense_solve(A, oracle.apply_bcs_vec(m.bcs, f))
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
