@@ -131,6 +131,7 @@ _SIGS = {
     "gf_bdf_assembler_data": [_P, _I, _P, _P, _P],
     "gf_bdf_component_descript": [_P, _I],
     "gf_bdf_element_descript": [_P, _I],
+    "gf_write_vtk": [ctypes.c_char_p, _I, _P, _I, _P, _P],
 }
 _RESTYPES = {"gf_context_last_error": ctypes.c_char_p,
              "gf_context_stream": _P, "gf_free": None,
@@ -271,6 +272,21 @@ def transpose_map(rowp, cols):
 def bandwidth(rowp, cols):
     rowp, cols = _i32(rowp), _i32(cols)
     return lib().gf_bandwidth(len(rowp) - 1, ptr(rowp), ptr(cols))
+
+
+def write_vtk(path, mesh, x=None):
+    """Legacy-VTK file of a mesh and its displacements x[6*nnodes] in the
+    unpermuted node order (printToVTK, include/mesh/vtk_writer.h:8-73)."""
+    xp = np.ascontiguousarray(mesh.xpts, dtype=np.float64)
+    cn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    xv = None
+    if x is not None:
+        xv = np.ascontiguousarray(x, dtype=np.float64)
+        if xv.shape != (6 * mesh.nnodes,):
+            raise ValueError(f"x must have {6 * mesh.nnodes} entries")
+    check(lib().gf_write_vtk(os.fsencode(path), mesh.nnodes, ptr(xp),
+                             mesh.nelems, ptr(cn),
+                             None if xv is None else ptr(xv)))
 
 
 class MeshLoader:

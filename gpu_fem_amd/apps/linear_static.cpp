@@ -10,7 +10,8 @@
 // set-up of examples/uCRM/linear_static.cu:13-80 (nxe is then ignored).
 //
 // usage: linear_static <nxe> <levFill> <rcm 0|1> <restart> <rtol> <out.bin>
-//        [--ccw] [--direct] [--bdf file]
+//        [--ccw] [--direct] [--bdf file] [--vtk file]
+// --vtk writes the mesh and the displacements as legacy VTK (printToVTK).
 // prints one JSON line; writes the unpermuted solution (6*nnodes doubles).
 #include <cstdio>
 #include <cstring>
@@ -23,7 +24,8 @@ using namespace gpufem;
 int main(int argc, char** argv) {
   if (argc < 7) {
     std::fprintf(stderr, "usage: %s nxe levFill rcm restart rtol out.bin "
-                         "[--ccw] [--direct] [--bdf file]\n", argv[0]);
+                         "[--ccw] [--direct] [--bdf file] [--vtk file]\n",
+                 argv[0]);
     return 2;
   }
   const int nxe = std::atoi(argv[1]), nye = nxe;
@@ -34,7 +36,9 @@ int main(int argc, char** argv) {
   const char* out = argv[6];
   bool ccw = false, direct = false;
   const char* bdf = nullptr;
+  const char* vtk = nullptr;
   for (int i = 7; i < argc; i++) {
+    if (!std::strcmp(argv[i], "--vtk") && i + 1 < argc) vtk = argv[++i];
     if (!std::strcmp(argv[i], "--ccw")) ccw = true;
     if (!std::strcmp(argv[i], "--direct")) direct = true;
     if (!std::strcmp(argv[i], "--bdf") && i + 1 < argc) bdf = argv[++i];
@@ -82,6 +86,7 @@ int main(int argc, char** argv) {
       assembler.apply_bcs(loads);
       solvers::direct_LU_solve(kmat, loads, soln);
       auto h = soln.createHostVec();
+      if (vtk) printToVTK(assembler, h, vtk);
       FILE* f = std::fopen(out, "wb");
       std::fwrite(h.getPtr(), sizeof(T), h.getSize(), f);
       std::fclose(f);
@@ -143,6 +148,7 @@ int main(int argc, char** argv) {
     bsr_post_solve<DeviceVec<T>>(kmat, loads, soln);
 
     auto h = soln.createHostVec();
+    if (vtk) printToVTK(assembler, h, vtk);
     FILE* f = std::fopen(out, "wb");
     std::fwrite(h.getPtr(), sizeof(T), h.getSize(), f);
     std::fclose(f);

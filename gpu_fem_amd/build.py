@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build", "gpufem_amd")
 LIB = os.path.join(HERE, "libgpufem_amd.so")
-SOURCES = ["symbolic.cpp", "amd.cpp", "bdf.cpp", "capi.cpp", "server.cpp", "assembly.hip", "bsr.hip",
+SOURCES = ["symbolic.cpp", "amd.cpp", "bdf.cpp", "vtk.cpp", "capi.cpp", "server.cpp", "assembly.hip", "bsr.hip",
            "ilu.hip", "krylov.hip"]
 HEADERS = ["common.h", "symbolic.h", "krylov.h", "mitc4.h"]
 ARCH = os.environ.get("GPUFEM_ARCH", "gfx950")

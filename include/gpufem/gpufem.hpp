@@ -850,6 +850,28 @@ Assembler createPlateAssembler(int nxe, int nye, double Lx, double Ly, double E,
                    physData);
 }
 
+// ------------------------------------------------------------ printToVTK
+// include/mesh/vtk_writer.h:8-73: mesh and displacement field (soln in the
+// unpermuted node order, host or device vector) as legacy VTK
+namespace detail {
+template <typename T>
+HostVec<T> to_host(const HostVec<T>& v) { return v; }
+template <typename T>
+HostVec<T> to_host(const DeviceVec<T>& v) { return v.createHostVec(); }
+}  // namespace detail
+
+template <class Assembler, class Vec>
+void printToVTK(Assembler assembler, Vec soln, std::string filename) {
+  auto x = assembler.getXpts().createHostVec();
+  auto c = assembler.getConn().createHostVec();
+  auto u = detail::to_host(soln);
+  if (u.getSize() != assembler.get_num_vars())
+    throw Error(GF_ERR_INVALID, "printToVTK: solution size");
+  check(gf_write_vtk(filename.c_str(), assembler.get_num_nodes(), x.getPtr(),
+                     assembler.get_num_elements(), c.getPtr(), u.getPtr()),
+        "printToVTK");
+}
+
 // point-sampled sine load on w (examples/plate_demo/_plate_utils.h:146-184)
 template <typename T, class Phys>
 std::vector<T> getPlateLoads(int nxe, int nye, double Lx, double Ly,

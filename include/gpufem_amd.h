@@ -308,6 +308,12 @@ int gf_bdf_assembler_data(gf_bdf b, int vars_per_node, int* elem_conn,
 const char* gf_bdf_component_descript(gf_bdf b, int comp);
 const char* gf_bdf_element_descript(gf_bdf b, int comp);
 
+/* ---- legacy-VTK output (printToVTK, include/mesh/vtk_writer.h:8-73; host
+ *      only): nodes, quads (corners written around the element) and the
+ *      displacement u v w of vars[6*nnodes] (NULL: zero) as "disp" ---- */
+int gf_write_vtk(const char* path, int nnodes, const double* xpts, int nelems,
+                 const int* conn, const double* vars);
+
 #ifdef __cplusplus
 }
 #endif

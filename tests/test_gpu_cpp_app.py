@@ -90,3 +90,23 @@ def test_bdf_mesh_direct_path_matches_dense():
     A = bsr_to_dense(s["krp"], s["kcp"], s["kv"])
     xd = oracle.dense_solve(A, oracle.apply_bcs_vec(m.bcs, f))
     assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+def test_vtk_output_holds_the_solution():
+    """--vtk (printToVTK, include/mesh/vtk_writer.h:8-73): the file's node
+    coordinates and displacement field are the mesh's and the app's own
+    solution, to the six significant digits the format keeps."""
+    print("[gpu] linear_static app --direct --vtk nx=6", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "plate.vtk")
+        info, x = run_app(6, 0, 0, 10, 1e-10, "--direct", "--vtk", p)
+        lines = open(p).read().splitlines()
+    m = mesh(6)
+    n = m.nnodes
+    i = lines.index(f"POINTS {n} double64")
+    pts = np.array([[float(v) for v in l.split()] for l in lines[i + 1:i + 1 + n]])
+    np.testing.assert_allclose(pts, m.xpts, rtol=1e-5, atol=1e-12)
+    j = lines.index("VECTORS disp double64")
+    u = np.array([[float(v) for v in l.split()] for l in lines[j + 1:j + 1 + n]])
+    ref = x.reshape(n, 6)[:, :3]
+    np.testing.assert_allclose(u, ref, rtol=1e-5, atol=1e-6 * np.abs(ref).max())
