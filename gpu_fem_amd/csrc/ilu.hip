@@ -635,8 +635,14 @@ __global__ void __launch_bounds__(256)
 
 
 constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
+// Batches of 10 blocks of a pass whose factor values are loaded into
+// registers before the row waits; blocks beyond them are loaded after the
+// wait, on the row's critical path.  400x400 plate, AMD + ILU(16), L+U pair:
+// 3 -> 4.33 ms, 4 -> 4.09, 6 -> 3.71, 8 -> 3.48, 10 -> 3.36, 12 -> 3.29
+// (FP64: 256 VGPRs and a few AGPRs, one wavefront per SIMD, which still
+// holds the three wavefronts per CU a sweep launches; no scratch).
 #ifndef GPUFEM_SWEEP_ITER
-#define GPUFEM_SWEEP_ITER 4
+#define GPUFEM_SWEEP_ITER 12
 #endif
 #ifndef GPUFEM_SWEEP_MINWAVES
 #define GPUFEM_SWEEP_MINWAVES 1
@@ -649,8 +655,11 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 #ifndef GPUFEM_SWEEP_CRIT
 #define GPUFEM_SWEEP_CRIT 0
 #endif
+// longest sleep between polls, x 64 cycles: L+U pair on the 400x400 plate
+// (ILU(16), 4 batches) 16 -> 4.14 ms, 8 -> 4.09, 4 -> 4.06, 2 -> 4.06,
+// 1 -> 4.07; with 12 batches 8 -> 3.29, 2 -> 3.26
 #ifndef GPUFEM_SWEEP_BACKOFF
-#define GPUFEM_SWEEP_BACKOFF 8  // longest sleep between polls, x 64 cycles
+#define GPUFEM_SWEEP_BACKOFF 2
 #endif
 // wavefronts per compute unit for a sweep launch: fewer waiting wavefronts
 // mean less polling and counter traffic; measured on the 400x400 plate
