@@ -664,7 +664,8 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 // wavefronts per compute unit for a sweep launch: fewer waiting wavefronts
 // mean less polling and counter traffic; measured on the 400x400 plate
 // (AMD + ILU(12)): 16 -> 4.37 ms per L+U pair, 4 -> 3.62, 3 -> 3.51,
-// 2 -> 3.64, 1 -> 5.17
+// 2 -> 3.64, 1 -> 5.17; with 12 register batches (ILU(16)): 2 -> 3.55,
+// 3 -> 3.26, 4 -> 3.31, 5 -> 3.61
 #ifndef GPUFEM_SWEEP_WAVES_PER_CU
 #define GPUFEM_SWEEP_WAVES_PER_CU 3
 #endif
