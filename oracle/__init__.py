@@ -292,9 +292,17 @@ def permute_vec(perm, v):
     return out
 
 
-def bsr_ilu(rowp, cols, vals, dtype=np.float64):
+def bsr_ilu(rowp, cols, vals, dtype=np.float64, rows=None):
+    """Block ILU in place on a copy; ``rows``: factor only rows [0, rows)
+    (FP64; a timing sample, exact for those rows)."""
     v = np.ascontiguousarray(vals, dtype=dtype).copy()
     n = len(rowp) - 1
+    if rows is not None:
+        L = lib()
+        L.orc_bsr_ilu_rows_f64.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p,
+                                           ctypes.c_int]
+        return v, L.orc_bsr_ilu_rows_f64(n, _i32(rowp), _i32(cols), v,
+                                         int(rows))
     if dtype == np.float64:
         st = lib().orc_bsr_ilu_f64(n, _i32(rowp), _i32(cols), v)
     else:

@@ -137,15 +137,16 @@ extern "C" void orc_permute_vec(int n, const int* perm, const double* in,
 namespace {
 
 template <typename T>
-int bsr_ilu(int n, const int* rowp, const int* cols, T* vals) {
-  // Row-oriented (IKJ) block ILU on the given pattern.
+int bsr_ilu(int n, const int* rowp, const int* cols, T* vals, int row_end) {
+  // Row-oriented (IKJ) block ILU on the given pattern; rows [0, row_end)
+  // only (a row depends on earlier rows alone, so a prefix is exact).
   std::vector<int> diag(n, -1);
   for (int i = 0; i < n; i++)
     for (int k = rowp[i]; k < rowp[i + 1]; k++)
       if (cols[k] == i) diag[i] = k;
   for (int i = 0; i < n; i++)
     if (diag[i] < 0) return -(i + 1);
-  for (int i = 0; i < n; i++) {
+  for (int i = 0; i < row_end; i++) {
     for (int kk = rowp[i]; kk < diag[i]; kk++) {
       int k = cols[kk];
       T* Aik = &vals[36 * (size_t)kk];
@@ -264,11 +265,16 @@ double nrm2(const std::vector<double>& v) {
 
 extern "C" int orc_bsr_ilu_f64(int n, const int* rowp, const int* cols,
                                double* vals) {
-  return bsr_ilu<double>(n, rowp, cols, vals);
+  return bsr_ilu<double>(n, rowp, cols, vals, n);
 }
 extern "C" int orc_bsr_ilu_f32(int n, const int* rowp, const int* cols,
                                float* vals) {
-  return bsr_ilu<float>(n, rowp, cols, vals);
+  return bsr_ilu<float>(n, rowp, cols, vals, n);
+}
+extern "C" int orc_bsr_ilu_rows_f64(int n, const int* rowp, const int* cols,
+                                    double* vals, int row_end) {
+  return bsr_ilu<double>(n, rowp, cols, vals,
+                         row_end < 0 ? 0 : (row_end > n ? n : row_end));
 }
 extern "C" void orc_bsr_spmv_f64(int n, const int* rowp, const int* cols,
                                  const double* vals, const double* x,

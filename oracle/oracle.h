@@ -75,6 +75,9 @@ void orc_permute_vec(int nnodes, const int* perm, const double* in,
 int orc_bsr_ilu_f64(int nnodes, const int* rowp, const int* cols,
                     double* vals);
 int orc_bsr_ilu_f32(int nnodes, const int* rowp, const int* cols, float* vals);
+/* rows [0, row_end) only: a timing sample (a row depends on earlier rows) */
+int orc_bsr_ilu_rows_f64(int nnodes, const int* rowp, const int* cols,
+                         double* vals, int row_end);
 void orc_bsr_spmv_f64(int nnodes, const int* rowp, const int* cols,
                       const double* vals, const double* x, double* y);
 void orc_bsr_trsv_f64(int nnodes, const int* rowp, const int* cols,
