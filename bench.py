@@ -87,7 +87,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="Poisson arrival rate (requests/s); 0: all at t=0")
     ap.add_argument("--serve-precision", type=int, default=2)
-    ap.add_argument("--serve-lev-fill", type=int, default=12)
+    ap.add_argument("--serve-lev-fill", type=int, default=32)
     ap.add_argument("--serve-stall", type=int, default=4,
                     help="stall_cycles of the served requests")
     ap.add_argument("--seed", type=int, default=0)
@@ -470,8 +470,10 @@ def serve_main(args):
     print(f"[serve] {len(reqs)} requests generated in {time.time()-t0:.1f}s; "
           f"nodes {min(m.nnodes for m in reqs)}..{max(m.nnodes for m in reqs)}",
           file=sys.stderr, flush=True)
-    # AMD + ILU(12): AMD + ILU(3) stagnates near a relative residual of 1 on
-    # the thinner plates and cylinders of this mix
+    # AMD + ILU(32) by default: AMD + ILU(3) stagnates near a relative
+    # residual of 1 on the thinner plates and cylinders of this mix, ILU(12)
+    # needs re-runs with more fill for the thinnest cylinders, and ILU(40)
+    # has rows over the factorisation's 1,024-block limit on the largest
     # stall_cycles: a solve whose residual falls by less than 10% over
     # --serve-stall restart cycles stops early (and is re-run with twice the
     # level of fill when it stopped far from the tolerance)
