@@ -18,7 +18,7 @@ BUILD = os.path.join(ROOT, "_build", "gpufem_amd")
 LIB = os.path.join(HERE, "libgpufem_amd.so")
 SOURCES = ["symbolic.cpp", "amd.cpp", "bdf.cpp", "vtk.cpp", "capi.cpp", "server.cpp", "assembly.hip", "bsr.hip",
            "ilu.hip", "krylov.hip"]
-HEADERS = ["common.h", "symbolic.h", "krylov.h", "mitc4.h"]
+HEADERS = ["common.h", "symbolic.h", "krylov.h", "mitc4.h", "problem.h"]
 ARCH = os.environ.get("GPUFEM_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -15,6 +15,7 @@
 
 #include "common.h"
 #include "krylov.h"
+#include "problem.h"
 #include "symbolic.h"
 
 namespace gf {
@@ -130,52 +131,30 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-int problem_setup(gf_problem p, const int* conn, const double* xpts,
-                  const double* data, const int* bcs) {
+// Uploads a plan and allocates the device work space (every device buffer
+// of a problem is allocated here; ProblemPlan::device_bytes mirrors the list)
+int problem_setup(gf_problem p, gf::ProblemPlan& pl, const int* conn,
+                  const double* xpts, const double* data, const int* bcs) {
   gf_context ctx = p->ctx;
   const int n = p->nnodes, ne = p->nelems;
-  std::vector<int> rp, cl;
-  GF_TRY(gf::node_graph(ne, n, 4, conn, rp, cl));
-  p->h_perm.resize(n);
-  if (p->opt.reorder == 1) {
-    GF_TRY(gf::rcm_order(n, rp.data(), cl.data(), 0, 1, p->h_perm.data()));
-  } else if (p->opt.reorder == 2) {
-    GF_TRY(gf::amd_order(n, rp.data(), cl.data(), p->h_perm.data()));
-  } else {
-    for (int i = 0; i < n; i++) p->h_perm[i] = i;
-  }
-  std::vector<int> krp, kcl, mrp, mcl;
-  GF_TRY(gf::pattern_permute(n, rp.data(), cl.data(), p->h_perm.data(), krp,
-                             kcl));
-  GF_TRY(gf::iluk_symbolic(n, krp.data(), kcl.data(), p->opt.lev_fill, mrp,
-                           mcl, nullptr));
-  p->nnzb_k = krp[n];
-  p->nnzb_m = mrp[n];
-  std::vector<int> emap((size_t)16 * ne), tr, tc, tm;
-  GF_TRY(gf::elem_ind_map(ne, 4, conn, krp.data(), kcl.data(),
-                          p->h_perm.data(), emap.data()));
-  GF_TRY(gf::transpose_map(n, krp.data(), kcl.data(), tr, tc, tm));
-  std::vector<int> iperm(n);
-  for (int i = 0; i < n; i++) iperm[p->h_perm[i]] = i;
-  for (int i = 0; i < p->nbcs; i++)
-    if (bcs[i] < 0 || bcs[i] >= 6 * n)
-      return gf::set_error(ctx, GF_ERR_INVALID, "bc %d out of range", bcs[i]);
-
+  p->h_perm = pl.perm;
+  p->nnzb_k = pl.krp[n];
+  p->nnzb_m = pl.mrp[n];
   GF_TRY(upload(ctx, &p->conn, conn, (size_t)4 * ne));
   GF_TRY(upload(ctx, &p->xpts, xpts, (size_t)3 * n));
   GF_TRY(upload(ctx, &p->data, data, (size_t)4 * ne));
   GF_TRY(upload(ctx, &p->bcs, bcs, (size_t)p->nbcs));
-  GF_TRY(upload(ctx, &p->perm, p->h_perm.data(), (size_t)n));
-  GF_TRY(upload(ctx, &p->iperm, iperm.data(), (size_t)n));
-  GF_TRY(upload(ctx, &p->krowp, krp.data(), krp.size()));
-  GF_TRY(upload(ctx, &p->kcols, kcl.data(), kcl.size()));
-  GF_TRY(upload(ctx, &p->emap, emap.data(), emap.size()));
-  GF_TRY(gf_asm_plan_create(ctx, ne, n, conn, emap.data(), p->nnzb_k,
+  GF_TRY(upload(ctx, &p->perm, pl.perm.data(), (size_t)n));
+  GF_TRY(upload(ctx, &p->iperm, pl.iperm.data(), (size_t)n));
+  GF_TRY(upload(ctx, &p->krowp, pl.krp.data(), pl.krp.size()));
+  GF_TRY(upload(ctx, &p->kcols, pl.kcl.data(), pl.kcl.size()));
+  GF_TRY(upload(ctx, &p->emap, pl.emap.data(), pl.emap.size()));
+  GF_TRY(gf_asm_plan_create(ctx, ne, n, conn, pl.emap.data(), p->nnzb_k,
                             &p->plan));
-  GF_TRY(upload(ctx, &p->trowp, tr.data(), tr.size()));
-  GF_TRY(upload(ctx, &p->tcols, tc.data(), tc.size()));
-  GF_TRY(upload(ctx, &p->tmap, tm.data(), tm.size()));
-  GF_TRY(gf_ilu_create(ctx, n, mrp.data(), mcl.data(), &p->ilu));
+  GF_TRY(upload(ctx, &p->trowp, pl.tr.data(), pl.tr.size()));
+  GF_TRY(upload(ctx, &p->tcols, pl.tc.data(), pl.tc.size()));
+  GF_TRY(upload(ctx, &p->tmap, pl.tm.data(), pl.tm.size()));
+  GF_TRY(gf_ilu_create(ctx, n, pl.mrp.data(), pl.mcl.data(), &p->ilu));
   const size_t N = 6 * (size_t)n;
   GF_TRY(dalloc(ctx, &p->kvals, 36 * (size_t)p->nnzb_k));
   GF_TRY(dalloc(ctx, &p->mvals, 36 * (size_t)p->nnzb_m));
@@ -201,6 +180,114 @@ int problem_setup(gf_problem p, const int* conn, const double* xpts,
 }
 
 }  // namespace
+
+namespace gf {
+
+bool options_valid(const gf_options& o) {
+  return o.restart >= 1 && o.restart <= 512 && o.lev_fill >= 0 &&
+         o.reorder >= 0 && o.reorder <= 2 && o.precision >= 0 &&
+         o.precision <= 2 && o.max_iters >= 0 && o.rtol > 0.0;
+}
+
+int problem_plan(int n, int ne, const int* conn, int nbcs, const int* bcs,
+                 const gf_options& opt, ProblemPlan& pl, std::string* why) {
+  auto fail = [&](int code, const char* msg) {
+    if (why) *why = msg;
+    return code;
+  };
+  for (size_t i = 0; i < 4 * (size_t)ne; i++)
+    if (conn[i] < 0 || conn[i] >= n)
+      return fail(GF_ERR_INVALID, "element node out of range");
+  for (int i = 0; i < nbcs; i++)
+    if (bcs[i] < 0 || bcs[i] >= 6 * n)
+      return fail(GF_ERR_INVALID, "bc dof out of range");
+  pl.nnodes = n;
+  pl.nelems = ne;
+  pl.nbcs = nbcs;
+  pl.opt = opt;
+  std::vector<int> rp, cl;
+  GF_TRY(node_graph(ne, n, 4, conn, rp, cl));
+  pl.perm.resize(n);
+  if (opt.reorder == 1) {
+    GF_TRY(rcm_order(n, rp.data(), cl.data(), 0, 1, pl.perm.data()));
+  } else if (opt.reorder == 2) {
+    GF_TRY(amd_order(n, rp.data(), cl.data(), pl.perm.data()));
+  } else {
+    for (int i = 0; i < n; i++) pl.perm[i] = i;
+  }
+  GF_TRY(pattern_permute(n, rp.data(), cl.data(), pl.perm.data(), pl.krp,
+                         pl.kcl));
+  GF_TRY(iluk_symbolic(n, pl.krp.data(), pl.kcl.data(), opt.lev_fill, pl.mrp,
+                       pl.mcl, nullptr));
+  pl.emap.resize((size_t)16 * ne);
+  GF_TRY(elem_ind_map(ne, 4, conn, pl.krp.data(), pl.kcl.data(),
+                      pl.perm.data(), pl.emap.data()));
+  GF_TRY(transpose_map(n, pl.krp.data(), pl.kcl.data(), pl.tr, pl.tc, pl.tm));
+  pl.iperm.resize(n);
+  for (int i = 0; i < n; i++) pl.iperm[pl.perm[i]] = i;
+  return GF_OK;
+}
+
+size_t ProblemPlan::device_bytes(bool caching) const {
+  const size_t n = nnodes, ne = nelems, N = 6 * n;
+  const size_t nk = krp.empty() ? 0 : krp[n], nm = mrp.empty() ? 0 : mrp[n];
+  const size_t m = opt.restart;
+  const bool mixed = opt.precision != 0;
+  size_t total = 0;
+  auto add = [&](size_t bytes) {
+    if (bytes == 0) bytes = 1;
+    total += caching ? cache_class(bytes) : bytes;
+  };
+  const size_t I = sizeof(int), D = sizeof(double), F = sizeof(float);
+  // mesh, permutation, K pattern, maps, assembly plan, transpose
+  add(4 * ne * I); add(3 * n * D); add(4 * ne * D); add(nbcs * I);
+  add(n * I); add(n * I); add((n + 1) * I); add(nk * I); add(16 * ne * I);
+  add(ne * I); add(ne * I);
+  add((n + 1) * I); add(nk * I); add(nk * I);
+  // ILU analysis: pattern, diagonal, level orders, chunk offsets (at most
+  // one chunk per row), row descriptors, pivot, counter, flags, done
+  add((n + 1) * I); add(nm * I); add(n * I); add(n * I); add(n * I);
+  add((n + 1) * I); add(4 * n * I); add(4 * n * I); add((n + 1) * I);
+  add(I); add(8); add(2 * I); add(n * I);
+  // values and vectors
+  add(36 * nk * D); add(36 * nm * D);
+  for (int i = 0; i < 5; i++) add(N * D);
+  if (mixed) {
+    if (opt.precision == 1) add(36 * nk * F);
+    add(36 * nm * F);
+    for (int i = 0; i < 3; i++) add(N * F);
+  }
+  // Krylov work space (KrylovWork::init)
+  add(N * (m + 1) * D);
+  for (int i = 0; i < 4; i++) add(N * D);
+  add((m + 2) * D); add((m + 2) * D); add(2 * D); add((m + 1) * D);
+  add(((m + 1) * m + 3 * m + 2) * D); add(256 * (2 * m + 4) * D);
+  add((m + 1) * (m + 1) * D); add((2 * m + 4) * D); add((2 * m + 4) * D);
+  add(D); add(4 * D); add(I);
+  if (mixed)
+    for (int i = 0; i < 3; i++) add(N * F);
+  return total;
+}
+
+int problem_create_planned(gf_context ctx, ProblemPlan& pl, const int* conn,
+                           const double* xpts, const double* data,
+                           const int* bcs, gf_problem* out) {
+  auto* p = new gf_problem_s;
+  p->ctx = ctx;
+  p->opt = pl.opt;
+  p->nnodes = pl.nnodes;
+  p->nelems = pl.nelems;
+  p->nbcs = pl.nbcs;
+  const int st = problem_setup(p, pl, conn, xpts, data, bcs);
+  if (st != GF_OK) {
+    gf_problem_destroy(p);
+    return st;
+  }
+  *out = p;
+  return GF_OK;
+}
+
+}  // namespace gf
 
 extern "C" {
 
@@ -228,28 +315,20 @@ int gf_problem_create(gf_context ctx, int nnodes, int nelems, const int* conn,
   if (!ctx || nnodes <= 0 || nelems <= 0 || !conn || !xpts || !data ||
       nbcs < 0 || (nbcs > 0 && !bcs) || !out)
     return gf::set_error(ctx, GF_ERR_INVALID, "problem_create: bad args");
-  auto* p = new gf_problem_s;
-  p->ctx = ctx;
+  gf_options o;
   if (opt)
-    p->opt = *opt;
+    o = *opt;
   else
-    gf_options_default(&p->opt);
-  if (p->opt.restart < 1 || p->opt.restart > 512 || p->opt.lev_fill < 0 ||
-      p->opt.reorder < 0 || p->opt.reorder > 2 || p->opt.precision < 0 ||
-      p->opt.precision > 2) {
-    delete p;
+    gf_options_default(&o);
+  if (!gf::options_valid(o))
     return gf::set_error(ctx, GF_ERR_INVALID, "problem_create: bad options");
-  }
-  p->nnodes = nnodes;
-  p->nelems = nelems;
-  p->nbcs = nbcs;
-  int st = problem_setup(p, conn, xpts, data, bcs);
-  if (st != GF_OK) {
-    gf_problem_destroy(p);
-    return st;
-  }
-  *out = p;
-  return GF_OK;
+  gf::ProblemPlan pl;
+  std::string why;
+  int st = gf::problem_plan(nnodes, nelems, conn, nbcs, bcs, o, pl, &why);
+  if (st != GF_OK)
+    return gf::set_error(ctx, st, "problem_create: %s",
+                         why.empty() ? "symbolic phase failed" : why.c_str());
+  return gf::problem_create_planned(ctx, pl, conn, xpts, data, bcs, out);
 }
 
 int gf_problem_assemble(gf_problem p) {
@@ -390,6 +469,12 @@ extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
       (!mixed && !mlu) || (mixed && !mlu32) ||
       (precision == 1 && !kvals32))
     return gf::set_error(ctx, GF_ERR_INVALID, "gmres_solve: bad args");
+  // the sweeps launch on the ILU handle's stream and the Krylov kernels on
+  // ctx's: both must be the same stream for the solve to be ordered
+  if (gf::ilu_context(ilu) != ctx)
+    return gf::set_error(ctx, GF_ERR_INVALID,
+                         "gmres_solve: the ILU handle belongs to another "
+                         "context");
   const size_t N = 6 * (size_t)nnodes;
   gf::KrylovWork kw;
   double *w64 = nullptr, *w64b = nullptr;

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "common.h"
+#include "krylov.h"
 #include "symbolic.h"
 
 struct gf_ilu_s {
@@ -1177,6 +1178,11 @@ int gf_ilu_check(gf_ilu h) {
 const int* gf_ilu_device_cols(gf_ilu h) { return h ? h->cols : nullptr; }
 
 }  // extern "C"
+
+namespace gf {
+int* ilu_sweep_err(gf_ilu h) { return h ? h->err : nullptr; }
+gf_context ilu_context(gf_ilu h) { return h ? h->ctx : nullptr; }
+}  // namespace gf
 
 namespace {
 template <typename T>

@@ -64,4 +64,9 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
                 int max_iters, int stall_cycles, double rtol, int prec,
                 GmresResult* res);
 
+// The sweeps' wait-expired flag (device int, set by a triangular sweep whose
+// bounded dependency wait ran out) and the context an ILU handle launches on.
+int* ilu_sweep_err(gf_ilu h);
+gf_context ilu_context(gf_ilu h);
+
 }  // namespace gf

@@ -548,6 +548,12 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
   GF_HIP(ctx, hipStreamSynchronize(s));
   double bnorm = W.host_scal[0];
   if (bnorm == 0.0) bnorm = 1.0;
+  // a sweep whose bounded wait expired sets this flag; it is cleared here so
+  // that a flag left by an earlier, abandoned solve is not reported now, and
+  // read at every cycle start, where the host synchronises anyway
+  int* sweep_err = op.ilu ? ilu_sweep_err(op.ilu) : nullptr;
+  int* host_err = (int*)(W.host_scal + 6);
+  if (sweep_err) GF_HIP(ctx, hipMemsetAsync(sweep_err, 0, sizeof(int), s));
   int it = 0, outer = 0, status = GF_ERR_NO_CONVERGENCE;
   double true_rel = 0.0, last_est = 0.0;
   std::vector<double> hist;  // true relative residual at each cycle start
@@ -560,9 +566,21 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     GF_TRY(W.dots(W.r, N, 1, W.r, W.nrm, 2));
     GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.nrm, sizeof(double),
                                hipMemcpyDeviceToHost, s));
+    *host_err = 0;
+    if (sweep_err)
+      GF_HIP(ctx, hipMemcpyAsync(host_err, sweep_err, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
     GF_HIP(ctx, hipStreamSynchronize(s));
     const double beta = W.host_scal[0];
     true_rel = beta / bnorm;
+    if (*host_err) {
+      // the preconditioner applications since the last cycle start used
+      // stand-in values: the iterate cannot be trusted
+      status = set_error(ctx, GF_ERR_TIMEOUT,
+                         "triangular sweep: a dependency wait expired");
+      GF_HIP(ctx, hipMemsetAsync(sweep_err, 0, sizeof(int), s));
+      break;
+    }
     if (verbose)
       fprintf(stderr, "gmres: cycle %d  iterations %d  true rel. residual %.3e"
               "  last estimate %.3e\n", outer, it, true_rel, last_est / bnorm);
@@ -707,10 +725,6 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     GF_TRY(op.apply_prec(W.tmp, W.z, prec));
     hipLaunchKernelGGL(axpy_k, grid1(N), dim3(256), 0, s, N, 1.0, W.z, x);
     GF_TRY(check_launch(ctx, "gmres update"));
-  }
-  if (op.ilu) {
-    const int ck = gf_ilu_check(op.ilu);
-    if (ck != GF_OK) status = ck;
   }
   if (res) {
     res->iterations = it;

@@ -14,10 +14,12 @@
 #include <condition_variable>
 #include <deque>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
 #include "common.h"
+#include "problem.h"
 
 namespace {
 
@@ -29,7 +31,6 @@ struct Request {
   std::vector<int> conn, bcs;
   std::vector<double> xpts, data, loads;
   gf_options opt{};
-  size_t bytes = 0;  // estimated device footprint
   Clock::time_point arrival;
 };
 
@@ -46,14 +47,19 @@ struct Result {
 struct gf_server_s {
   int device = 0;
   size_t budget = 0;
-  std::atomic<size_t> in_use{0};
+  // device bytes counted against the budget: the reservation of every
+  // request being solved plus the device blocks each worker context keeps
+  // cached between requests
+  size_t in_use = 0;
   std::mutex mu;
   std::condition_variable cv;      // queue / budget changes
   std::deque<Request> queue;
   std::deque<Result> done;
   std::vector<std::thread> workers;
   bool stopping = false;
-  int active = 0;
+  int active = 0;    // requests taken from the queue and not yet finished
+  int busy = 0;      // workers holding a request's reservation
+  int pressure = 0;  // workers waiting for budget: idle workers drop caches
   size_t peak_bytes = 0;
   long long completed = 0, failed = 0, escalated = 0;
   std::vector<double> latencies;
@@ -62,58 +68,100 @@ struct gf_server_s {
 
 namespace {
 
-// device bytes of one problem: K and ILU(k) values (ILU(k) fill estimated as
-// (1 + 0.4 k) x nnzb_K, capped at 8x; AMD + ILU(12) on plates measures 5.3x),
-// FP32 copies in mixed mode, the Krylov basis, index arrays and mesh
-size_t estimate_bytes(const Request& r) {
-  const double nnzb_k = 9.0 * r.nnodes;
-  const double fill = std::min(8.0, 1.0 + 0.4 * r.opt.lev_fill);
-  double b = nnzb_k * 288.0 * (1.0 + fill);
-  if (r.opt.precision == 1) b *= 1.5;
-  b += 48.0 * r.nnodes * (r.opt.restart + 8);
-  b += nnzb_k * 12.0 + r.nelems * 96.0 + r.nnodes * 64.0;
-  return (size_t)(b * 1.1);
+// A lower bound of a request's device bytes, known before its symbolic
+// phase: the diagonal blocks of K and of the preconditioner and the Krylov
+// basis.  A request above the budget even by this bound can never be run.
+size_t lower_bound_bytes(const Request& r) {
+  const double n = r.nnodes;
+  return (size_t)(2.0 * 288.0 * n + 48.0 * n * (r.opt.restart + 1));
 }
 
 constexpr int kMaxEscalations = 2;
 constexpr double kEscalateAbove = 1e3;
+constexpr int kMaxEscalatedFill = 64;
 
+// One worker: takes the oldest queued request, runs its symbolic phase on
+// the host (which fixes its device footprint exactly), waits until that
+// footprint fits the budget, then runs the device path on its own stream.
 void worker_main(gf_server_s* s) {
   gf_context ctx = nullptr;
   if (gf::context_create(s->device, true, &ctx) != GF_OK) return;
+  size_t held = 0;  // this worker's share of in_use
+  // give this worker's cached blocks back (hipFree) and its share of the
+  // budget with them; called with the lock held, drops it meanwhile
+  auto drop_cache = [&](std::unique_lock<std::mutex>& lk) {
+    lk.unlock();
+    gf::release_cache(ctx);
+    lk.lock();
+    s->in_use -= held;
+    held = 0;
+    s->cv.notify_all();
+  };
   for (;;) {
     Request req;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      // admit the oldest request once its footprint fits the budget
-      s->cv.wait(lk, [&] {
-        return s->stopping ||
-               (!s->queue.empty() &&
-                (s->in_use + s->queue.front().bytes <= s->budget ||
-                 s->active == 0));
-      });
+      for (;;) {
+        s->cv.wait(lk, [&] {
+          return s->stopping || !s->queue.empty() ||
+                 (s->pressure > 0 && held > 0);
+        });
+        if (!s->queue.empty() || s->stopping) break;
+        drop_cache(lk);  // another worker waits for budget
+      }
       if (s->stopping && s->queue.empty()) break;
       req = std::move(s->queue.front());
       s->queue.pop_front();
-      s->in_use += req.bytes;
       s->active++;
-      s->peak_bytes = std::max(s->peak_bytes, (size_t)s->in_use);
     }
     Result res;
     res.id = req.id;
-    const auto t_start = Clock::now();
-    res.queue_s = std::chrono::duration<double>(t_start - req.arrival).count();
+    res.queue_s = std::chrono::duration<double>(Clock::now() - req.arrival).count();
     // A request whose solve ends far from the tolerance (relative residual
     // above kEscalateAbove * rtol: the preconditioner failed, not FP64's
     // attainable accuracy) or whose factorisation or GMRES breaks down is
     // run again with twice the ILU level of fill (up to kMaxEscalations
-    // times): on thin shells a higher level converges where ILU(12) does not
+    // times, and not beyond ILU(kMaxEscalatedFill)); each attempt gives its
+    // reservation back and is admitted again with its own footprint.
     int st = GF_OK;
     for (int attempt = 0;; attempt++) {
+      gf::ProblemPlan plan;
+      std::string why;
+      st = gf::problem_plan(req.nnodes, req.nelems, req.conn.data(),
+                            (int)req.bcs.size(), req.bcs.data(), req.opt, plan,
+                            &why);
+      if (st != GF_OK) break;
+      const size_t bytes = plan.device_bytes(true);
+      if (bytes > s->budget) {
+        st = GF_ERR_OVER_BUDGET;
+        break;
+      }
+      {
+        // admission: this worker's footprint becomes its cache plus the
+        // request (cached blocks may be reused, so this is an upper bound)
+        std::unique_lock<std::mutex> lk(s->mu);
+        for (;;) {
+          const size_t others = s->in_use - held;
+          if (others + held + bytes <= s->budget) break;
+          if (held > 0) {
+            drop_cache(lk);
+            continue;
+          }
+          if (s->busy == 0 && others == 0) break;
+          s->pressure++;
+          s->cv.notify_all();
+          s->cv.wait(lk);
+          s->pressure--;
+        }
+        s->in_use += bytes;
+        held += bytes;
+        s->busy++;
+        s->peak_bytes = std::max(s->peak_bytes, s->in_use);
+      }
       gf_problem p = nullptr;
-      st = gf_problem_create(ctx, req.nnodes, req.nelems, req.conn.data(),
-                             req.xpts.data(), req.data.data(),
-                             (int)req.bcs.size(), req.bcs.data(), &req.opt, &p);
+      st = gf::problem_create_planned(ctx, plan, req.conn.data(),
+                                      req.xpts.data(), req.data.data(),
+                                      req.bcs.data(), &p);
       if (st == GF_OK) st = gf_problem_assemble(p);
       if (st == GF_OK) st = gf_problem_factor(p);
       if (st == GF_OK) {
@@ -121,17 +169,25 @@ void worker_main(gf_server_s* s) {
         st = gf_problem_solve(p, req.loads.data(), res.x.data(), 0, &res.info);
       }
       if (p) gf_problem_destroy(p);
+      {
+        // the problem's blocks went back to the context's cache
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->in_use -= held;
+        held = ctx->cached_bytes;
+        s->in_use += held;
+        s->busy--;
+      }
+      s->cv.notify_all();
       const bool numerical =
           (st == GF_ERR_NO_CONVERGENCE &&
            !(res.info.rel_resid <= kEscalateAbove * req.opt.rtol)) ||
           st == GF_ERR_ZERO_PIVOT || st == GF_ERR_BREAKDOWN;
-      if (!numerical || attempt >= kMaxEscalations) break;
-      const size_t before = req.bytes;
-      req.opt.lev_fill = std::min(2 * std::max(req.opt.lev_fill, 1), 64);
-      req.bytes = estimate_bytes(req);
+      if (!numerical || attempt >= kMaxEscalations ||
+          req.opt.lev_fill >= kMaxEscalatedFill)
+        break;
+      req.opt.lev_fill =
+          std::min(2 * std::max(req.opt.lev_fill, 1), kMaxEscalatedFill);
       std::lock_guard<std::mutex> lk(s->mu);
-      s->in_use += req.bytes - before;
-      s->peak_bytes = std::max(s->peak_bytes, (size_t)s->in_use);
       s->escalated++;
     }
     res.status = st;
@@ -139,7 +195,6 @@ void worker_main(gf_server_s* s) {
         std::chrono::duration<double>(Clock::now() - req.arrival).count();
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      s->in_use -= req.bytes;
       s->active--;
       if (st == GF_OK)
         s->completed++;
@@ -149,6 +204,10 @@ void worker_main(gf_server_s* s) {
       s->done.push_back(std::move(res));
     }
     s->cv.notify_all();
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->in_use -= held;
   }
   gf_context_destroy(ctx);
 }
@@ -197,11 +256,11 @@ int gf_server_submit(gf_server s, long long id, int nnodes, int nelems,
     r.opt = *opt;
   else
     gf_options_default(&r.opt);
-  r.bytes = estimate_bytes(r);
+  if (!gf::options_valid(r.opt)) return GF_ERR_INVALID;
   r.arrival = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    if (r.bytes > s->budget) return GF_ERR_OVER_BUDGET;
+    if (lower_bound_bytes(r) > s->budget) return GF_ERR_OVER_BUDGET;
     s->queue.push_back(std::move(r));
   }
   s->cv.notify_all();
