@@ -69,6 +69,10 @@ def lib():
         L.orc_transpose.argtypes = [ctypes.c_int, _i32p, _i32p, _pp, _pp, _pp]
         L.orc_bandwidth.argtypes = [ctypes.c_int, _i32p, _i32p]
         L.orc_min_degree.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p]
+        L.orc_min_degree_stack.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p]
+        L.orc_md_replay.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
+                                    _i32p]
+        L.orc_md_replay.restype = ctypes.c_longlong
         L.orc_assemble_jacobian.argtypes = [
             ctypes.c_int, _i32p, _f64p, ctypes.c_void_p, _f64p, _i32p,
             ctypes.c_void_p, _f64p, ctypes.c_void_p, ctypes.c_int]
@@ -87,6 +91,8 @@ def lib():
                                        _f64p, _f64p]
         L.orc_bsr_trsv_f64.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p,
                                        _f64p, _f64p]
+        L.orc_bsr_resid_comp_f64.argtypes = [ctypes.c_int, _i32p, _i32p,
+                                             _f64p, _f64p, _f64p, _f64p]
         L.orc_bsr_trsv_f32.argtypes = [ctypes.c_int, _i32p, _i32p, _f32p,
                                        _f32p, _f32p]
         L.orc_gmres.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _i32p,
@@ -231,6 +237,24 @@ def min_degree(rowp, cols):
     return out
 
 
+def min_degree_stack(rowp, cols):
+    """Exact minimum degree, ties to the most recently updated node."""
+    n = len(rowp) - 1
+    out = np.zeros(n, np.int32)
+    lib().orc_min_degree_stack(n, _i32(rowp), _i32(cols), out)
+    return out
+
+
+def md_replay(rowp, cols, perm):
+    """Replays an ordering on the explicit elimination graph: (exact degree
+    of each pivot, least exact degree at that step, nnz(L) blocks)."""
+    n = len(rowp) - 1
+    dp = np.zeros(n, np.int32)
+    dm = np.zeros(n, np.int32)
+    nnz = lib().orc_md_replay(n, _i32(rowp), _i32(cols), _i32(perm), dp, dm)
+    return dp, dm, int(nnz)
+
+
 # ---------------------------------------------------------------- numeric
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
@@ -315,6 +339,14 @@ def bsr_spmv(rowp, cols, vals, x):
     lib().orc_bsr_spmv_f64(len(rowp) - 1, _i32(rowp), _i32(cols), _f64(vals),
                            _f64(x), y)
     return y
+
+
+def bsr_resid_comp(rowp, cols, vals, x, b):
+    """b - A x with compensated (twice-FP64) accumulation."""
+    r = np.zeros(6 * (len(rowp) - 1))
+    lib().orc_bsr_resid_comp_f64(len(rowp) - 1, _i32(rowp), _i32(cols),
+                                 _f64(vals), _f64(x), _f64(b), r)
+    return r
 
 
 def bsr_trsv(rowp, cols, lu, b, dtype=np.float64):

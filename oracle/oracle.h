@@ -51,6 +51,13 @@ int orc_bandwidth(int nnodes, const int* rowp, const int* cols);
 int orc_full_fill(int nnodes, const int* rowp, const int* cols, int** frowp,
                   int** fcols);
 int orc_min_degree(int nnodes, const int* rowp, const int* cols, int* perm);
+/* replay of an ordering on the explicit elimination graph: exact degree of
+   each pivot and least exact degree at that step; returns nnz(L) blocks */
+long long orc_md_replay(int nnodes, const int* rowp, const int* cols,
+                        const int* perm, int* dpiv, int* dmin);
+/* exact minimum degree, ties to the most recently updated node */
+int orc_min_degree_stack(int nnodes, const int* rowp, const int* cols,
+                         int* perm);
 
 /* ---- assembly / matrix ops (include/linalg/bsr_mat.h, .cuh, vec.h, .cuh) */
 void orc_assemble_jacobian(int nelems, const int* conn, const double* xpts,
@@ -80,6 +87,11 @@ int orc_bsr_ilu_rows_f64(int nnodes, const int* rowp, const int* cols,
                          double* vals, int row_end);
 void orc_bsr_spmv_f64(int nnodes, const int* rowp, const int* cols,
                       const double* vals, const double* x, double* y);
+/* r = b - A x accumulated with error-free transformations (TwoProduct by
+   FMA, TwoSum), rounded once */
+void orc_bsr_resid_comp_f64(int nnodes, const int* rowp, const int* cols,
+                            const double* vals, const double* x,
+                            const double* b, double* r);
 void orc_bsr_trsv_f64(int nnodes, const int* rowp, const int* cols,
                       const double* lu, const double* b, double* x);
 void orc_bsr_trsv_f32(int nnodes, const int* rowp, const int* cols,

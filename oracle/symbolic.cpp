@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <cstdint>
 #include <vector>
 
 #include "oracle.h"
@@ -240,6 +241,104 @@ extern "C" int orc_min_degree(int n, const int* rowp, const int* cols,
       if (!done[a] && adj[best][a])
         for (int b = 0; b < n; b++)
           if (b != a && !done[b] && adj[best][b]) adj[a][b] = adj[b][a] = 1;
+  }
+  return n;
+}
+
+// Minimum-degree checks on the explicit elimination graph (dense bitsets;
+// small graphs only).  The reference takes its AMD ordering from the absent
+// sparse-utils library, so an ordering is checked against what minimum
+// degree means (Rose's elimination-graph model; Tinney-Walker scheme 2)
+// rather than against a reference permutation.
+namespace {
+struct ElimGraph {
+  int n, words;
+  std::vector<uint64_t> adj, alive;
+  ElimGraph(int n_, const int* rowp, const int* cols)
+      : n(n_), words((n_ + 63) / 64), adj((size_t)n_ * ((n_ + 63) / 64), 0),
+        alive((n_ + 63) / 64, 0) {
+    for (int i = 0; i < n; i++) {
+      alive[i / 64] |= 1ull << (i % 64);
+      for (int k = rowp[i]; k < rowp[i + 1]; k++) {
+        const int j = cols[k];
+        if (j == i) continue;
+        row(i)[j / 64] |= 1ull << (j % 64);
+        row(j)[i / 64] |= 1ull << (i % 64);
+      }
+    }
+  }
+  uint64_t* row(int i) { return &adj[(size_t)i * words]; }
+  // number of uneliminated neighbours
+  int degree(int i) {
+    int d = 0;
+    for (int w = 0; w < words; w++) d += __builtin_popcountll(row(i)[w] & alive[w]);
+    return d;
+  }
+  // eliminates p: its uneliminated neighbours become a clique; returns them
+  std::vector<int> eliminate(int p) {
+    alive[p / 64] &= ~(1ull << (p % 64));
+    std::vector<uint64_t> nb(words);
+    std::vector<int> out;
+    for (int w = 0; w < words; w++) nb[w] = row(p)[w] & alive[w];
+    for (int w = 0; w < words; w++)
+      for (uint64_t b = nb[w]; b; b &= b - 1) out.push_back(64 * w + __builtin_ctzll(b));
+    for (int a : out) {
+      for (int w = 0; w < words; w++) row(a)[w] |= nb[w];
+      row(a)[a / 64] &= ~(1ull << (a % 64));
+    }
+    return out;
+  }
+};
+}  // namespace
+
+// Replays perm (perm[old] = new): dpiv[k] = exact degree of the k-th pivot
+// when it is eliminated, dmin[k] = the least exact degree of any node left
+// at that step.  Returns the blocks of the complete factor's lower triangle
+// including the diagonal, sum_k (dpiv[k] + 1).
+extern "C" long long orc_md_replay(int n, const int* rowp, const int* cols,
+                                   const int* perm, int* dpiv, int* dmin) {
+  ElimGraph g(n, rowp, cols);
+  std::vector<int> iperm(n);
+  for (int i = 0; i < n; i++) iperm[perm[i]] = i;
+  long long nnz = 0;
+  for (int k = 0; k < n; k++) {
+    int best = n;
+    for (int i = 0; i < n; i++)
+      if (g.alive[i / 64] >> (i % 64) & 1) best = std::min(best, g.degree(i));
+    const int p = iperm[k];
+    const int d = g.degree(p);
+    if (dpiv) dpiv[k] = d;
+    if (dmin) dmin[k] = best;
+    nnz += d + 1;
+    g.eliminate(p);
+  }
+  return nnz;
+}
+
+// Exact minimum degree with the tie-break of a bucket-stack implementation:
+// among the nodes of least exact degree, the one whose degree was updated
+// most recently; nodes updated by the same elimination in descending index
+// order, nodes never updated in ascending index order (the buckets are
+// filled from the highest index down, and an update pushes a node on top of
+// its bucket).  Every neighbour of an eliminated node counts as updated.
+extern "C" int orc_min_degree_stack(int n, const int* rowp, const int* cols,
+                                    int* perm) {
+  ElimGraph g(n, rowp, cols);
+  std::vector<long long> stamp(n);
+  for (int i = 0; i < n; i++) stamp[i] = n - 1 - i;
+  long long clock = n;
+  for (int k = 0; k < n; k++) {
+    int best = -1, bd = 0;
+    for (int i = 0; i < n; i++) {
+      if (!(g.alive[i / 64] >> (i % 64) & 1)) continue;
+      const int d = g.degree(i);
+      if (best < 0 || d < bd || (d == bd && stamp[i] > stamp[best])) {
+        best = i;
+        bd = d;
+      }
+    }
+    perm[best] = k;
+    for (int a : g.eliminate(best)) stamp[a] = clock++;
   }
   return n;
 }

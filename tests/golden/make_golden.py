@@ -11,7 +11,13 @@ Fixtures (all numeric data, no reference source text):
       computeILUk (k = 0..3) and get_elem_ind_map compiled from
       include/linalg/reordering_utils.h and include/linalg/bsr_utils.h by
       oracle/build_ref.sh, on small seeded meshes (plates and a cylinder)
+  iluk_highfill_ref.npz  the reference's computeILUk at the levels of fill
+      the solver runs (k = 6, 12, 16, 32, 56) on AMD- and RCM-ordered
+      patterns of mid-size meshes: input pattern, output row pointers and
+      level counts per row, and the SHA-256 of the output column and level
+      arrays (int32, little-endian)
 """
+import hashlib
 import os
 import re
 import sys
@@ -66,7 +72,53 @@ def symbolic():
     np.savez_compressed(os.path.join(HERE, "symbolic_ref.npz"), **data)
 
 
+HIGHFILL_K = (6, 12, 16, 32, 56)
+
+
+def highfill_meshes():
+    from gpu_fem_amd import meshes as M
+    return {"plate40_amd": (M.plate(40, 40), "amd"),
+            "plate100_amd": (M.plate(100, 100), "amd"),
+            "plate30_rcm": (M.plate(30, 30), "rcm"),
+            "cyl24x16_amd": (M.cylinder(24, 16, seed=11), "amd"),
+            "wingbox_amd": (M.wingbox(nc=10, nh=3, nsb=2, nribs=4, seed=5),
+                            "amd")}
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, "<i4").tobytes()).hexdigest()
+
+
+def highfill():
+    import gpu_fem_amd as G
+    import oracle
+    assert oracle.ref() is not None, "build oracle/_ref first"
+    data = {}
+    for name, (m, order) in highfill_meshes().items():
+        rp, cp = oracle.ref_node_graph(m.conn, m.nnodes)
+        # the input ordering only has to be fixed: the product's AMD or the
+        # reference's RCM, stored with the pattern
+        perm = G.amd_order(rp, cp) if order == "amd" else oracle.ref_rcm(rp, cp)
+        prp, pcp = oracle.permute_pattern(rp, cp, perm)
+        data[f"{name}/rowp"] = prp
+        data[f"{name}/cols"] = pcp
+        full = None
+        for k in HIGHFILL_K + (10 ** 6,):
+            frp, fcp, flv = oracle.ref_iluk(prp, pcp, k, fill=100.0)
+            if k == 10 ** 6:
+                full = frp[-1]
+                continue
+            data[f"{name}/ilu{k}_rowp"] = frp
+            data[f"{name}/ilu{k}_rowlev"] = np.add.reduceat(
+                flv, frp[:-1]).astype(np.int64) if len(flv) else flv
+            data[f"{name}/ilu{k}_sha_cols"] = np.array(sha(fcp))
+            data[f"{name}/ilu{k}_sha_levs"] = np.array(sha(flv))
+        data[f"{name}/full_nnzb"] = np.array([full])
+    np.savez_compressed(os.path.join(HERE, "iluk_highfill_ref.npz"), **data)
+
+
 if __name__ == "__main__":
     kelem()
     symbolic()
+    highfill()
     print("golden fixtures written to", HERE)

@@ -24,7 +24,17 @@ def oracle_problem(mesh, reorder="rcm", k=0):
     if reorder == "rcm":
         perm = oracle.rcm(rp, cp)
     elif reorder == "amd":
-        perm = G.amd_order(rp, cp)  # ordering itself is checked by invariants
+        # The reference's AMD lives in the absent sparse-utils library, so
+        # there is no reference permutation to compare with.  The product's
+        # ordering is used on both sides once the oracle has checked it is a
+        # minimum-degree ordering: replayed on the explicit elimination graph,
+        # every pivot has the least exact degree left at its step (the
+        # replay is cubic in the node count: meshes above 2,500 nodes rely on
+        # the same check in tests/test_oracle_symbolic.py on smaller meshes)
+        perm = G.amd_order(rp, cp)
+        if n <= 2500:
+            dp, dm, _ = oracle.md_replay(rp, cp, perm)
+            assert np.array_equal(dp, dm), "AMD pivot above the minimum degree"
     else:
         perm = np.arange(n, dtype=np.int32)
     krp, kcp = oracle.permute_pattern(rp, cp, perm)

@@ -111,3 +111,100 @@ def test_stagnating_request_is_rerun_with_more_fill(gpu_ctx):
     assert info.status == 0 and info.rel_resid <= 1e-9
     assert st["escalated"] == 1 and st["completed"] == 1
     assert np.linalg.norm(x - x2) <= 1e-10 * np.linalg.norm(x2)
+
+
+def served_mix_sample(seed=5, n=6):
+    """The low end of the served request mix (gpu_fem_amd.meshes.
+    served_requests): plates n x n with n in [32, 40] and 64 x 128
+    cylinders (R = 1 m, L = 3 m), thickness uniform in [1, 10] mm,
+    uniform-plus-sinusoidal pressure of magnitude uniform in [0.5, 5] kPa,
+    clamped or simply supported with probability 1/2."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        t = float(rng.uniform(1e-3, 10e-3))
+        p0 = float(rng.uniform(500.0, 5000.0))
+        bc = "clamped" if rng.random() < 0.5 else "ss"
+        if i % 2 == 0:
+            m = int(rng.integers(32, 41))
+            mesh = M.plate(m, m, 1.0, 1.0, thick=t, bc=bc, load="uniform_sine",
+                           load_mag=p0)
+        else:
+            mesh = M.cylinder(64, 128, t_range=(t, t),
+                              seed=int(rng.integers(1 << 30)), pressure=p0,
+                              axial_line_load=0.0, bc=bc)
+        out.append(mesh)
+    return out
+
+
+# the served configuration: AMD + ILU(32), GMRES(200) to 1e-8, FP32 ILU and
+# sweeps inside FP64 GMRES, stall window of 4 cycles
+SERVED = dict(reorder=2, lev_fill=32, restart=200, rtol=1e-8, max_iters=20000,
+              stall_cycles=4, precision=2)
+
+
+def test_served_mix_against_fp64_oracle(gpu_ctx):
+    """Requests of the served mix arrive as a Poisson stream (rate 20/s) at
+    a server with 4 slots; every served result has a true relative residual
+    at or below 1e-8 evaluated by the oracle in compensated FP64, lies close
+    to the oracle's FP64 solution (solved to 1e-10 or its floor), and equals
+    the same request solved alone."""
+    import time
+
+    import oracle
+    from gpu_helpers import oracle_problem
+
+    reqs = served_mix_sample()
+    print(f"[gpu] served mix sample: {[m.name for m in reqs]}", flush=True)
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, len(reqs)))
+    srv = G.Server(budget_gb=200.0, slots=4)
+    t0 = time.perf_counter()
+    for i, m in enumerate(reqs):
+        while time.perf_counter() - t0 < arrivals[i]:
+            time.sleep(0.001)
+        assert srv.submit(i, m, **SERVED) == 0
+    srv.drain()
+    got = {}
+    while True:
+        r = srv.poll()
+        if r is None:
+            break
+        got[r[0]] = r
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and len(got) == len(reqs)
+    for i, m in enumerate(reqs):
+        _, x, info, lat = got[i]
+        assert info.status == 0 and info.rel_resid <= 1e-8
+        # alone, same configuration
+        p = G.Problem(gpu_ctx, m, **SERVED)
+        p.assemble()
+        p.factor()
+        xa, ia = p.solve()
+        perm = p.perm()
+        p.close()
+        assert np.linalg.norm(x - xa) <= 1e-10 * np.linalg.norm(xa)
+        # FP64 oracle: same ordering (checked as a minimum-degree ordering on
+        # the smaller meshes), ILU(32) in FP64, GMRES to 1e-10
+        s = oracle_problem(m, "amd", SERVED["lev_fill"])
+        np.testing.assert_array_equal(perm, s["perm"])
+        lu, ok = oracle.bsr_ilu(s["mrp"], s["mcp"], s["mv"])
+        assert ok == 0
+        b = oracle.permute_vec(s["perm"],
+                               oracle.apply_bcs_vec(m.bcs, m.loads))
+        xo, _, oi = oracle.gmres(s["krp"], s["kcp"], s["kv"], s["mrp"],
+                                 s["mcp"], lu, b, restart=200, rtol=1e-10,
+                                 max_iters=600)
+        xp = oracle.permute_vec(s["perm"], x)
+        r = oracle.bsr_resid_comp(s["krp"], s["kcp"], s["kv"], xp, b)
+        tr = np.linalg.norm(r) / np.linalg.norm(b)
+        err = np.linalg.norm(xp - xo) / np.linalg.norm(xo)
+        print(f"    {m.name}: latency {lat * 1e3:.0f} ms, iters "
+              f"{info.iterations}, residual {info.rel_resid:.2e} (oracle "
+              f"{tr:.2e}), |x - x_oracle_fp64| / |x_oracle_fp64| = {err:.2e} "
+              f"(oracle residual {oi['rel_resid']:.1e})", flush=True)
+        assert tr <= 1.01e-8
+        # a residual of 1e-8 bounds the displacement error by the condition
+        # number; these shells stay within 1e-6 of the FP64 oracle
+        assert err <= 1e-6

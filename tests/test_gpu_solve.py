@@ -2,12 +2,16 @@
 the oracle (same symbolic phase, assembly, ILU and GMRES restated on the CPU)
 and against a direct solve.
 
-Tolerances: index arrays bit-exact; displacement within 1e-8 relative
-(2-norm) of the oracle's FP64 solution; the true relative residual
-||b - K x|| / ||b|| evaluated by the oracle in FP64 at or below the requested
-tolerance (1e-10 for the small plate, 1e-8 otherwise) for both the FP64 and
-the mixed-precision path.
+Tolerances: index arrays bit-exact; both sides solve to the tightest relative
+residual FP64 reaches on the case (1e-12, or 1e-10 where x's own rounding
+leaves a residual floor near 3e-11), so that the displacement agreement
+measures the solvers and not the stopping point; the displacement of every
+precision mode (FP64, GMRES-IR, FP32 ILU inside FP64 GMRES) within 1e-8
+relative (2-norm) of the oracle's FP64 solution, and the true relative
+residual ||b - K x|| / ||b|| evaluated by the oracle with compensated FP64
+at or below the tolerance.
 """
+import functools
 import sys
 
 import numpy as np
@@ -31,7 +35,7 @@ def true_rel_resid(s, mesh, x):
     b = oracle.apply_bcs_vec(mesh.bcs, mesh.loads)
     bp = oracle.permute_vec(s["perm"], b)
     xp = oracle.permute_vec(s["perm"], x)
-    r = bp - oracle.bsr_spmv(s["krp"], s["kcp"], s["kv"], xp)
+    r = oracle.bsr_resid_comp(s["krp"], s["kcp"], s["kv"], xp, bp)
     return np.linalg.norm(r) / np.linalg.norm(bp)
 
 
@@ -48,36 +52,44 @@ def oracle_solution(s, mesh, restart, rtol):
 
 
 CASES = [
-    # name, mesh builder, reorder id, k, restart, rtol
+    # name, mesh builder, reorder id, k, restart, rtol (the tightest relative
+    # residual FP64 reaches on the case: the 5 mm plate and the wingbox have
+    # a floor near 3e-11, where the rounding of x itself leaves a residual)
     ("plate10_clamped_ilu0_natural",
      lambda: M.plate(10, 10, 1.0, 1.0, thick=0.01, bc="clamped", load="uniform"),
-     0, 0, 50, 1e-10),
+     0, 0, 50, 1e-12),
     ("plate24_ss_rcm_ilu3",
-     lambda: M.plate(24, 24, thick=0.005, bc="ss", load="sine"), 1, 3, 50, 1e-8),
+     lambda: M.plate(24, 24, thick=0.005, bc="ss", load="sine"), 1, 3, 50, 1e-10),
     ("cylinder_amd_ilu2",
-     lambda: M.cylinder(24, 16, seed=11), 2, 2, 50, 1e-8),
+     lambda: M.cylinder(24, 16, seed=11), 2, 2, 50, 1e-12),
     # skins, spars and ribs meeting at shared edge nodes (T-junctions)
     ("wingbox_amd_ilu3",
-     lambda: M.wingbox(nc=10, nh=3, nsb=2, nribs=4, seed=5), 2, 3, 50, 1e-8),
+     lambda: M.wingbox(nc=10, nh=3, nsb=2, nribs=4, seed=5), 2, 3, 50, 1e-10),
 ]
+CASE = {c[0]: c for c in CASES}
 
 
-@pytest.mark.parametrize("name,build,reorder,k,restart,rtol", CASES,
-                         ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("precision", [0, 1, 2], ids=["fp64", "mixed", "mixed_pc"])
-def test_solve_matches_oracle(gpu_ctx, name, build, reorder, k, restart, rtol,
-                              precision):
-    say(f"[gpu] solve {name} precision={precision}")
+@functools.lru_cache(maxsize=None)
+def oracle_case(name):
+    """Oracle problem and FP64 oracle solution of a case (shared by the
+    three precision modes)."""
+    _, build, reorder, k, restart, rtol = CASE[name]
     mesh = build()
+    s = oracle_problem(mesh, {0: "natural", 1: "rcm", 2: "amd"}[reorder], k)
+    x_o, info_o = oracle_solution(s, mesh, restart, rtol)
+    assert true_rel_resid(s, mesh, x_o) <= rtol * 1.01
+    return mesh, s, x_o, info_o
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+@pytest.mark.parametrize("precision", [0, 1, 2], ids=["fp64", "mixed", "mixed_pc"])
+def test_solve_matches_oracle(gpu_ctx, name, precision):
+    _, _, reorder, k, restart, rtol = CASE[name]
+    say(f"[gpu] solve {name} precision={precision} rtol={rtol:g}")
+    mesh, s, x_o, info_o = oracle_case(name)
     p = G.Problem(gpu_ctx, mesh, reorder=reorder, lev_fill=k, restart=restart,
-                  rtol=rtol, max_iters=20000, precision=precision)
-    perm = p.perm()
-    s = oracle_problem(mesh, {0: "natural", 1: "rcm", 2: "natural"}[reorder], k)
-    if reorder == 2:
-        # use the product's AMD permutation on the oracle side as well
-        rp, cp = oracle.node_graph(mesh.conn, mesh.nnodes)
-        s = _oracle_with_perm(mesh, rp, cp, perm, k)
-    np.testing.assert_array_equal(perm, s["perm"])
+                  rtol=rtol, max_iters=40000, precision=precision)
+    np.testing.assert_array_equal(p.perm(), s["perm"])
     # index arrays bit-exact
     _, nk, nm = p.sizes()
     assert nk == s["krp"][-1] and nm == s["mrp"][-1]
@@ -85,28 +97,15 @@ def test_solve_matches_oracle(gpu_ctx, name, build, reorder, k, restart, rtol,
     p.factor()
     x, info = p.solve()
     assert info.status == 0, info.status
-    x_o, info_o = oracle_solution(s, mesh, restart, rtol)
     rel = np.linalg.norm(x - x_o) / np.linalg.norm(x_o)
     tr = true_rel_resid(s, mesh, x)
     say(f"    iters={info.iterations} outer={info.outer} "
-        f"rel_resid={info.rel_resid:.3e} true={tr:.3e} |x-x_oracle|={rel:.3e} "
+        f"rel_resid={info.rel_resid:.3e} true={tr:.3e} "
+        f"|x-x_oracle_fp64|/|x_oracle_fp64|={rel:.3e} "
         f"oracle_iters={info_o['iters']}")
     assert tr <= rtol * 1.01
-    assert rel <= 1e-8 or (rtol >= 1e-8 and rel <= 1e-6)
+    assert rel <= 1e-8
     p.close()
-
-
-def _oracle_with_perm(mesh, rp, cp, perm, k):
-    krp, kcp = oracle.permute_pattern(rp, cp, perm)
-    emap = oracle.elem_ind_map(mesh.conn, krp, kcp, perm)
-    kv, _ = oracle.assemble_jacobian(mesh.conn, mesh.xpts, mesh.data, emap,
-                                     perm, int(krp[-1]), with_res=False)
-    tr, tc, tm = oracle.transpose(krp, kcp)
-    kv = oracle.apply_bcs_mat(mesh.bcs, krp, kcp, tr, tc, tm, perm, kv)
-    mrp, mcp, _ = oracle.iluk(krp, kcp, k)
-    mv = oracle.copy_values(krp, kcp, kv, mrp, mcp)
-    return dict(perm=np.asarray(perm, np.int32), krp=krp, kcp=kcp, kv=kv,
-                mrp=mrp, mcp=mcp, mv=mv, emap=emap, tr=tr, tc=tc, tm=tm)
 
 
 def test_problem_device_arrays_bit_exact(gpu_ctx):
