@@ -366,9 +366,13 @@ class Context:
     """A device + HIP stream (gf_context)."""
 
     def __init__(self, device=0):
+        import weakref
+
         h = _P()
         check(lib().gf_context_create(device, ctypes.byref(h)))
         self.h = h
+        # objects whose device memory belongs to this context: closed first
+        self._deps = weakref.WeakSet()
 
     def check(self, code):
         return check(code, self.h)
@@ -382,6 +386,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for d in list(self._deps):
+                d.close()
             lib().gf_context_destroy(self.h)
             self.h = None
 
@@ -402,6 +408,7 @@ class Ilu:
         ctx.check(lib().gf_ilu_create(ctx.h, len(rowp) - 1, ptr(self.rowp),
                                       ptr(self.cols), ctypes.byref(h)))
         self.h = h
+        ctx._deps.add(self)
 
     def info(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -461,6 +468,7 @@ class Problem:
             ctx.h, mesh.nnodes, mesh.nelems, ptr(conn), ptr(xpts), ptr(data),
             len(bcs), ptr(bcs), ctypes.byref(self.opt), ctypes.byref(h)))
         self.h = h
+        ctx._deps.add(self)
 
     def sizes(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
