@@ -90,6 +90,7 @@ _SIGS = {
     "gf_bsr_resid_f64": [_P, _I, _P, _P, _P, _P, _P,
                          ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(ctypes.c_double)],
+    "gf_bsr_resid_f64_noalloc": [_P, _I, _P, _P, _P, _P, _P, _P],
     "gf_ilu_create": [_P, _I, _P, _P, ctypes.POINTER(_P)],
     "gf_ilu_destroy": [_P],
     "gf_ilu_info": [_P, _PI, _PI, _PI],

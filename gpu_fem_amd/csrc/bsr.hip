@@ -6,6 +6,9 @@
 //   copy_pattern_kernel               bsr_mat.cuh:107-140
 //   permute_kernel                    vec.cuh:16-34
 //   spmv_kernel                       cusparseDbsrmv (gmres.cu:588-599)
+#include <mutex>
+#include <vector>
+
 #include "common.h"
 
 namespace gf {
@@ -245,6 +248,46 @@ __global__ void convert_kernel(size_t n, const double* __restrict__ in,
 }  // namespace
 }  // namespace gf
 
+// Scratch of gf_bsr_resid_f64_noalloc: the per-workgroup partial results
+// (largest |r| entry, its index, sum of squares) of at most
+// kResidScratchGroups workgroups of 40 block rows, i.e. meshes of up to
+// 40 * 2^17 = 5,242,880 nodes; 3 MiB per device, held until the process
+// ends.  Each device has its own copy at its own address, looked up once per
+// device with hipGetSymbolAddress (the array's name is not a device address
+// in host code).
+constexpr int kResidScratchGroups = 1 << 17;
+static __device__ __align__(256) double scratch[3 * kResidScratchGroups];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // per device, null until looked up
+
+// the device address of `scratch` on `device` (looked up on first use)
+int scratch_on_device(gf_context ctx, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr.empty()) {
+    int ndev = 0;
+    GF_HIP(ctx, hipGetDeviceCount(&ndev));
+    g_scratch_ptr.assign(ndev > 0 ? ndev : 1, nullptr);
+  }
+  if (ctx->device < 0 || ctx->device >= (int)g_scratch_ptr.size())
+    return gf::set_error(ctx, GF_ERR_DEVICE, "no such device");
+  double*& p = g_scratch_ptr[ctx->device];
+  if (!p) {
+    int cur = 0;
+    GF_HIP(ctx, hipGetDevice(&cur));
+    GF_HIP(ctx, hipSetDevice(ctx->device));
+    void* addr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    hipSetDevice(cur);
+    GF_TRY(gf::check_hip(ctx, e, "hipGetSymbolAddress(scratch)"));
+    p = (double*)addr;
+  }
+  *out = p;
+  return GF_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int gf_bc_apply_mat(gf_context ctx, int nbcs, const int* bcs, const int* rowp,
@@ -345,6 +388,33 @@ int gf_bsr_resid_f64(gf_context ctx, int nnodes, const int* rowp,
   if (max_entry) *max_entry = h[0];
   if (norm2) *norm2 = h[1];
   return GF_OK;
+}
+
+// gf_bsr_resid_f64 without allocation: the partial results live in the
+// library's static device array `scratch`, the two results go to device
+// memory, nothing synchronises.  Calls that overlap on one device share the
+// array and race.
+int gf_bsr_resid_f64_noalloc(gf_context ctx, int nnodes, const int* rowp,
+                             const int* cols, const double* vals,
+                             const double* x, const double* b, double* d_out) {
+  if (!ctx || nnodes <= 0 || !rowp || !cols || !vals || !x || !b || !d_out)
+    return gf::set_error(ctx, GF_ERR_INVALID, "bsr_resid_noalloc: bad args");
+  const long long nblk = (nnodes + 39LL) / 40;
+  if (nblk > kResidScratchGroups)
+    return gf::set_error(ctx, GF_ERR_INVALID,
+                         "bsr_resid_noalloc: %d nodes exceed the scratch "
+                         "limit of %d", nnodes, 40 * kResidScratchGroups);
+  double* base = nullptr;
+  GF_TRY(scratch_on_device(ctx, &base));
+  double* pmax = base;
+  long long* pidx = (long long*)(base + nblk);
+  double* pss = base + 2 * nblk;
+  hipLaunchKernelGGL(gf::resid_partial_kernel, dim3((unsigned)nblk), dim3(256),
+                     0, ctx->stream, nnodes, rowp, cols, vals, x, b, pmax,
+                     pidx, pss);
+  hipLaunchKernelGGL(gf::resid_final_kernel, dim3(1), dim3(256), 0,
+                     ctx->stream, (int)nblk, pmax, pidx, pss, d_out);
+  return gf::check_launch(ctx, "resid_kernel (noalloc)");
 }
 
 int gf_bsr_resid_comp_f64(gf_context ctx, int nnodes, const int* rowp,

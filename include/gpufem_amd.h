@@ -168,6 +168,20 @@ int gf_bsr_resid_f64(gf_context ctx, int nnodes, const int* d_rowp,
                      const double* d_x, const double* d_b, double* max_entry,
                      double* norm2);
 
+/* gf_bsr_resid_f64 without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The partial
+ * results live in a static device array of the library (3 MiB per device,
+ * held until the process ends): meshes of at most 5,242,880 nodes
+ * (GF_ERR_INVALID above).  d_out[0] = the entry of A x - b of largest
+ * magnitude (signed, first index on ties), d_out[1] = ||A x - b||_2, in
+ * device memory, stream-ordered; nothing synchronises.  Calls that overlap
+ * on one device share the array and race. */
+int gf_bsr_resid_f64_noalloc(gf_context ctx, int nnodes, const int* d_rowp,
+                             const int* d_cols, const double* d_vals,
+                             const double* d_x, const double* d_b,
+                             double* d_out);
+
 /* ---- ILU analysis, numeric factorisation and triangular sweeps
  *      (cusparseXbsrilu02 / cusparseXbsrsv2, gmres.cu:374-465, 609-625) ---- */
 /* host pattern in, uploads the pattern and the level schedules */
