@@ -516,6 +516,217 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 MFMA Jacobian: four elements per 64-lane wavefront, one per 16-lane
+// block of v_mfma_f64_4x4x4_4b_f64 (four independent 4x4x4 products per
+// instruction).  K_e = sum_q detXd_q B_q^T C B_q is one 24 x 24 x 36 product
+// with the 36 = 9 strains x 4 quadrature points stacked strain-major: k-block
+// s (4 wide) holds strain s at the four points.  Lane (i, q) of an element
+// (its A row i and k index q, see mfma_d_row) forms the columns j = i + 4J
+// (J = 0..5) of B_q in registers -- exactly the MFMA A operand
+// A[i][q] = B_q[s][4I + i] of tile row I and the B operand
+// D[q][j] = detXd_q (C B_q)[s][4J + j] of tile column J -- so B and D never
+// pass through LDS.  Only the 21 tiles on and above the diagonal of the
+// 6 x 6 tile grid are formed (K_e is symmetric): 9 x 21 MFMAs per four
+// elements, 24,192 flops per element.  K_e is then staged in LDS and
+// written as whole 6 x 6 blocks, consecutive lanes on consecutive entries
+// of a block: plain stores (first element of the colour order to touch the
+// block) and FP64 adds without return.  In the colour-ordered launches no
+// two elements of a launch share a node, so each add is ordered by the
+// launch sequence and the sum is deterministic; the uncoloured entry point
+// adds in any order.
+constexpr int kMfmaElems = 4;  // elements per wavefront (MFMA blocks)
+
+struct ElemMfma {
+  double X[12];
+  double u[24];
+  double data[4];
+  union {
+    struct {
+      mitc4::Geom g;
+      mitc4::BColumnPrep prep[24];  // quadrature-independent column parts
+    };
+    double Ke[24][25];  // padded rows
+  };
+};
+
+// Lane layout of v_mfma_f64_4x4x4_4b_f64 on gfx950 (measured): the block of
+// lane L is (L / 4) % 4; its A operand is A[L % 4][L / 16], its B operand
+// B[L / 16][L % 4] and its result D[L / 16][L % 4].  With the index
+// l16 = L % 4 + 4 (L / 16) inside the block, a lane's A row, B column and
+// D column are l16 % 4 and its k index and D row l16 / 4.
+__device__ __forceinline__ int mfma_d_row(int l16) { return l16 >> 2; }
+__device__ __forceinline__ int mfma_d_col(int l16) { return l16 & 3; }
+
+// Row s of detXd C (stress of a2d/a2disotropicshellstress.h:13-51) applied
+// to the strain column e[0..8].
+struct StressRows {
+  double Q0, Q1, G, am, ab, ad, As;
+  __device__ __forceinline__ void init(const double d[4]) {
+    const double E = d[0], nu = d[1], t = d[2], to = d[3];
+    const double D = E / (1.0 - nu * nu);
+    Q0 = D;
+    Q1 = nu * D;
+    G = E / 2.0 / (1.0 + nu);
+    am = t;
+    ab = -t * t * to;
+    ad = t * t * t / 12.0 + t * t * t * to * to;
+    As = (5.0 / 6.0) * t * G;
+  }
+  __device__ __forceinline__ double row(int s, const double e[9]) const {
+    switch (s) {
+      case 0: return am * (Q0 * e[0] + Q1 * e[1]) + ab * (Q0 * e[3] + Q1 * e[4]);
+      case 1: return am * (Q1 * e[0] + Q0 * e[1]) + ab * (Q1 * e[3] + Q0 * e[4]);
+      case 2: return am * (G * e[2]) + ab * (G * e[5]);
+      case 3: return ab * (Q0 * e[0] + Q1 * e[1]) + ad * (Q0 * e[3] + Q1 * e[4]);
+      case 4: return ab * (Q1 * e[0] + Q0 * e[1]) + ad * (Q1 * e[3] + Q0 * e[4]);
+      case 5: return ab * (G * e[2]) + ad * (G * e[5]);
+      case 6: return As * e[6];
+      case 7: return As * e[7];
+      default: return 10.0 * As * e[8];
+    }
+  }
+};
+
+#ifndef GPUFEM_MFMA_MINWAVES
+#define GPUFEM_MFMA_MINWAVES 2
+#endif
+template <bool kColored>
+__global__ void __launch_bounds__(64, GPUFEM_MFMA_MINWAVES)
+    jacobian_mfma_kernel(int nelems, const int* __restrict__ elem_list,
+                         const int* __restrict__ first_mask,
+                         const int* __restrict__ conn,
+                         const double* __restrict__ xpts,
+                         const double* __restrict__ vars,
+                         const double* __restrict__ data,
+                         const int* __restrict__ elem_ind_map,
+                         const int* __restrict__ perm, double* kvals,
+                         double* res) {
+  __shared__ ElemMfma S[kMfmaElems];
+  __shared__ int s_blk[kMfmaElems][16];
+  __shared__ int s_mask[kMfmaElems];
+  const int lane = threadIdx.x;
+  // MFMA block (element) of the lane and its index 0..15 inside the block
+  const int grp = (lane >> 2) & 3, l16 = (lane & 3) | ((lane >> 4) << 2);
+  const int slot = blockIdx.x * kMfmaElems + grp;
+  const bool valid = slot < nelems;
+  const int e = valid ? (kColored ? elem_list[slot] : slot) : 0;
+  ElemMfma& s = S[grp];
+  if (valid) {
+    for (int v = l16; v < 40; v += 16) {
+      if (v < 12) {
+        s.X[v] = xpts[3 * (size_t)conn[4 * (size_t)e + v / 3] + v % 3];
+      } else if (v < 36) {
+        const int w = v - 12;
+        s.u[w] = vars ? vars[6 * (size_t)conn[4 * (size_t)e + w / 6] + w % 6]
+                      : 0.0;
+      } else {
+        s.data[v - 36] = data[4 * (size_t)e + (v - 36)];
+      }
+    }
+    s_blk[grp][l16] = elem_ind_map[16 * (size_t)e + l16];
+    if (l16 == 0) s_mask[grp] = kColored ? first_mask[slot] : 0;
+  }
+  __syncthreads();
+  if (valid) {
+    if (l16 < 4)
+      mitc4::geom_node(l16, s.X, s.g);
+    else if (l16 < 9)
+      mitc4::geom_tying_x(l16 - 4, s.X, s.g);
+  }
+  __syncthreads();
+  if (valid) {
+    if (l16 < 4)
+      mitc4::geom_quad(l16, s.X, s.g);
+    else if (l16 < 9)
+      mitc4::geom_tying_n(l16 - 4, s.g);
+  }
+  __syncthreads();
+  // the quadrature-independent part of each column, once per column
+  if (valid)
+    for (int j = l16; j < 24; j += 16) mitc4::b_column_prep(j, s.g, s.prep[j]);
+  __syncthreads();
+  // B_q columns j = i + 4J of this lane's quadrature point q, all 9 strains
+  const int i = l16 & 3, q = (l16 >> 2) & 3;
+  double Bv[6][9];
+  StressRows C{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double sc = 0.0;
+  if (valid) {
+    C.init(s.data);
+    sc = s.g.scale[q];
+#pragma unroll
+    for (int J = 0; J < 6; J++)
+      mitc4::b_column_q(i + 4 * J, q, s.g, s.prep[i + 4 * J], Bv[J]);
+  } else {
+    // a block past the last element: zeros in, results discarded
+#pragma unroll
+    for (int J = 0; J < 6; J++)
+#pragma unroll
+      for (int k = 0; k < 9; k++) Bv[J][k] = 0.0;
+  }
+  // 21 tiles (I <= J) of the 6 x 6 grid of 4 x 4 tiles
+  double acc[21];
+#pragma unroll
+  for (int t = 0; t < 21; t++) acc[t] = 0.0;
+#pragma unroll
+  for (int st = 0; st < 9; st++) {
+    double dv[6];
+#pragma unroll
+    for (int J = 0; J < 6; J++) dv[J] = sc * C.row(st, Bv[J]);
+    int t = 0;
+#pragma unroll
+    for (int I = 0; I < 6; I++)
+#pragma unroll
+      for (int J = I; J < 6; J++, t++)
+        acc[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(Bv[I][st], dv[J], acc[t],
+                                                    0, 0, 0);
+  }
+  __syncthreads();  // geometry no longer read: Ke reuses its LDS
+  if (valid) {
+    const int dr = mfma_d_row(l16), dc = mfma_d_col(l16);
+    int t = 0;
+#pragma unroll
+    for (int I = 0; I < 6; I++)
+#pragma unroll
+      for (int J = I; J < 6; J++, t++) {
+        s.Ke[4 * I + dr][4 * J + dc] = acc[t];
+        if (J != I) s.Ke[4 * J + dc][4 * I + dr] = acc[t];
+      }
+  }
+  __syncthreads();
+  // whole 6 x 6 blocks: entry f of the wavefront's 4 x 16 x 36 block values
+#pragma unroll 4
+  for (int f = lane; f < kMfmaElems * 576; f += 64) {
+    const int g = f / 576, rem = f % 576, ab = rem / 36, rc = rem % 36;
+    if (blockIdx.x * kMfmaElems + g >= nelems) continue;
+    const int a = ab >> 2, b = ab & 3, r = rc / 6, c = rc % 6;
+    const double v = S[g].Ke[6 * a + r][6 * b + c];
+    double* d = kvals + 36 * (size_t)s_blk[g][ab] + rc;
+    // colour-ordered: no other element of this launch touches the block,
+    // so the add without return is ordered by the launches alone (the same
+    // sum, bit for bit, as a read-add-write, without the read's latency)
+    if (kColored && ((s_mask[g] >> ab) & 1))
+      *d = v;
+    else
+      unsafeAtomicAdd(d, v);
+  }
+  if (!res) return;  // uniform across the workgroup
+  if (valid) {
+    for (int row = l16; row < 24; row += 16) {
+      double v = 0.0;
+#pragma unroll
+      for (int c = 0; c < 24; c++) v += s.Ke[row][c] * s.u[c];
+      const int node = conn[4 * (size_t)e + row / 6];
+      const int prow = perm ? perm[node] : node;
+      double* d = &res[6 * (size_t)prow + row % 6];
+      if (!kColored)
+        unsafeAtomicAdd(d, v);
+      else
+        *d = ((s_mask[grp] >> (16 + row / 6)) & 1) ? v : *d + v;
+    }
+  }
+}
+
 // Residual only: R_e = sum_q D_q^T (B_q u) (C symmetric).
 __global__ void __launch_bounds__(64)
     residual_kernel(int nelems, const int* __restrict__ conn,
@@ -585,14 +796,18 @@ __global__ void __launch_bounds__(64)
 }  // namespace gf
 
 namespace {
-// GPUFEM_ASSEMBLY=element selects the one-element-per-wavefront kernel,
-// =batched the six-elements-per-wavefront one (atomic entry point only)
+// Jacobian kernel: the FP64 MFMA kernel by default; GPUFEM_ASSEMBLY=scalar
+// selects the two-elements-per-wavefront scalar kernel (jacobian2_kernel),
+// =element the one-element-per-wavefront kernel, =batched the
+// six-elements-per-wavefront one (atomic entry point only)
+enum { kAsmMfma = 0, kAsmElement = 1, kAsmBatched = 2, kAsmScalar = 3 };
 int assembly_variant() {
   static const int v = [] {
     const char* e = getenv("GPUFEM_ASSEMBLY");
-    if (e && std::string(e) == "element") return 1;
-    if (e && std::string(e) == "batched") return 2;
-    return 0;
+    if (e && std::string(e) == "element") return (int)kAsmElement;
+    if (e && std::string(e) == "batched") return (int)kAsmBatched;
+    if (e && std::string(e) == "scalar") return (int)kAsmScalar;
+    return (int)kAsmMfma;
   }();
   return v;
 }
@@ -609,14 +824,22 @@ int gf_assemble_jacobian_mitc4_f64(gf_context ctx, int nelems, int nnodes,
       !elem_ind_map || !kvals)
     return gf::set_error(ctx, GF_ERR_INVALID, "assemble_jacobian: bad args");
   if (nelems == 0) return GF_OK;
-  if (assembly_variant() == 1) {
+  if (assembly_variant() == kAsmMfma) {
+    hipLaunchKernelGGL(gf::jacobian_mfma_kernel<false>,
+                       dim3(gf::div_up(nelems, gf::kMfmaElems)), dim3(64), 0,
+                       ctx->stream, nelems, (const int*)nullptr,
+                       (const int*)nullptr, conn, xpts, vars, data,
+                       elem_ind_map, perm, kvals, res);
+    return gf::check_launch(ctx, "jacobian_mfma_kernel");
+  }
+  if (assembly_variant() == kAsmElement) {
     hipLaunchKernelGGL(gf::jacobian_kernel<false>, dim3(nelems), dim3(64), 0,
                        ctx->stream, nelems, (const int*)nullptr,
                        (const int*)nullptr, conn, xpts, vars, data,
                        elem_ind_map, perm, kvals, res);
     return gf::check_launch(ctx, "jacobian_kernel");
   }
-  if (assembly_variant() == 0) {
+  if (assembly_variant() == kAsmScalar) {
     hipLaunchKernelGGL(gf::jacobian2_kernel<false>,
                        dim3(gf::div_up(nelems, 2)), dim3(64), 0, ctx->stream,
                        nelems, (const int*)nullptr, (const int*)nullptr, conn,
@@ -719,7 +942,13 @@ int gf_assemble_jacobian_colored_f64(gf_context ctx, gf_asm_plan pl,
                                ctx->stream));
   for (int c = 0; c < pl->ncolors; c++) {
     const int c0 = pl->color_ptr[c], cnt = pl->color_ptr[c + 1] - c0;
-    if (assembly_variant() == 1)
+    if (assembly_variant() == kAsmMfma)
+      hipLaunchKernelGGL(gf::jacobian_mfma_kernel<true>,
+                         dim3(gf::div_up(cnt, gf::kMfmaElems)), dim3(64), 0,
+                         ctx->stream, cnt, pl->elems + c0, pl->mask + c0,
+                         conn, xpts, vars, data, elem_ind_map, perm, kvals,
+                         res);
+    else if (assembly_variant() == kAsmElement)
       hipLaunchKernelGGL(gf::jacobian_kernel<true>, dim3(cnt), dim3(64), 0,
                          ctx->stream, cnt, pl->elems + c0, pl->mask + c0,
                          conn, xpts, vars, data, elem_ind_map, perm, kvals,
