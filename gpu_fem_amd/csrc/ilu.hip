@@ -63,6 +63,9 @@ struct gf_ilu_s {
   // dataflow factorisation: done[i] == epoch once row i is final
   int* done = nullptr;
   int epoch = 0;
+  // dependency level of every chunk (host; batched sweeps merge the chunk
+  // lists of several systems level by level)
+  std::vector<int> lchunk_level, uchunk_level;
 };
 
 namespace gf {
@@ -732,15 +735,30 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // fill_next (lower sweep only, may be null): the entries of the row in the
 // upper sweep's output are set to the sentinel once the row's rhs is read,
 // which allows that output to alias the rhs.
-template <typename T, bool kLower, int kRows>
+//
+// kBatched: one launch sweeps the systems of several solves.  Chunk kc of
+// the launch is chunk merged[kc].y of system merged[kc].x, whose arrays are
+// in slots[merged[kc].x]; the merged list takes the systems' chunks level by
+// level, so every row waited on is still in an earlier chunk.
+template <typename T, bool kLower, int kRows, bool kBatched = false>
 __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
-    trsv_syncfree_kernel(const int4* __restrict__ meta,
-                         const int* __restrict__ cols,
-                         const int* __restrict__ chunk_ptr, int nchunks,
-                         const T* __restrict__ lu, const T* b, T* x,
-                         T* fill_next, unsigned long long* counter,
-                         unsigned long long base, int* err) {
+    trsv_syncfree_kernel(const int4* __restrict__ meta_,
+                         const int* __restrict__ cols_,
+                         const int* __restrict__ chunk_ptr_, int nchunks,
+                         const T* __restrict__ lu_, const T* b_, T* x_,
+                         T* fill_next_, unsigned long long* counter,
+                         unsigned long long base, int* err,
+                         const SweepSlot<T>* __restrict__ slots = nullptr,
+                         const int2* __restrict__ merged = nullptr) {
   using B = typename Sentinel<T>::Bits;
+  const int4* __restrict__ meta = meta_;
+  const int* __restrict__ cols = cols_;
+  const int* __restrict__ chunk_ptr = chunk_ptr_;
+  const T* __restrict__ lu = lu_;
+  const T* b = b_;
+  T* x = x_;
+  T* fill_next = fill_next_;
+  int cur_slot = -1;
   constexpr int G = 64 / kRows;        // lanes per row
   constexpr int kS = G / 6;            // block slots per row (10 or 5)
   constexpr int kH = kGather / G;      // dependency blocks per lane per pass
@@ -766,7 +784,29 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
     // take the next chunk now: the counter round trip overlaps this chunk
     unsigned long long tn = 0;
     if (lane == 0) tn = atomicAdd(counter, 1ull);
-    const int p0 = chunk_ptr[kc], p1 = chunk_ptr[kc + 1];
+    int lc = (int)kc;
+    if constexpr (kBatched) {
+      const int2 mc = merged[kc];
+      lc = mc.y;
+      if (*slots[mc.x].go != 1) {  // this system takes no step now
+        tn = __shfl(tn, 0, 64);
+        kc = (long long)(tn - base);
+        continue;
+      }
+      if (mc.x != cur_slot) {
+        cur_slot = mc.x;
+        const SweepSlot<T>& S = slots[mc.x];
+        meta = kLower ? S.lmeta : S.umeta;
+        chunk_ptr = kLower ? S.lchunk : S.uchunk;
+        cols = S.cols;
+        lu = S.lu;
+        b = kLower ? S.b : S.tmp;
+        x = kLower ? S.tmp : S.x;
+        fill_next = kLower ? S.x : nullptr;
+        cid = -1;  // the cached row belongs to another system
+      }
+    }
+    const int p0 = chunk_ptr[lc], p1 = chunk_ptr[lc + 1];
     int4 mnext = (p0 + grp < p1) ? meta[p0 + grp] : make_int4(0, 0, 0, 0);
     for (int pb = p0; pb < p1; pb += kRows) {
       // rows pb + grp of one level (independent when kRows > 1)
@@ -1037,6 +1077,19 @@ int gf_ilu_create(gf_context ctx, int n, const int* rowp, const int* cols,
   if (h->uchains) uch = ucn; else chunks(h->uptr, uch);
   h->nlchunk = (int)lch.size() - 1;
   h->nuchunk = (int)uch.size() - 1;
+  if (!use_chains) {
+    auto levels_of = [](const std::vector<int>& ch, const std::vector<int>& lp,
+                        std::vector<int>& out) {
+      out.resize(ch.size() - 1);
+      size_t l = 0;
+      for (size_t c = 0; c + 1 < ch.size(); c++) {
+        while (l + 2 < lp.size() && lp[l + 1] <= ch[c]) l++;
+        out[c] = (int)l;
+      }
+    };
+    levels_of(lch, h->lptr, h->lchunk_level);
+    levels_of(uch, h->uptr, h->uchunk_level);
+  }
   // row descriptors in hand-out order: one 16-byte load per row
   // {row, first block, end block, critical dependency}: the diagonal block
   // is the end (L) or one before the first (U); the critical dependency is
@@ -1180,8 +1233,59 @@ const int* gf_ilu_device_cols(gf_ilu h) { return h ? h->cols : nullptr; }
 }  // extern "C"
 
 namespace gf {
+unsigned long long sweep_sentinel_bits(bool fp64) {
+  return fp64 ? Sentinel<double>::value : Sentinel<float>::value;
+}
 int* ilu_sweep_err(gf_ilu h) { return h ? h->err : nullptr; }
 gf_context ilu_context(gf_ilu h) { return h ? h->ctx : nullptr; }
+
+int ilu_view(gf_ilu h, IluView* v) {
+  if (!h || !v || h->lchains || h->uchains) return GF_ERR_INVALID;
+  v->n = h->n;
+  v->lmeta = h->lmeta;
+  v->umeta = h->umeta;
+  v->lchunk = h->lchunk;
+  v->uchunk = h->uchunk;
+  v->cols = h->cols;
+  v->lchunk_level = &h->lchunk_level;
+  v->uchunk_level = &h->uchunk_level;
+  v->grid = h->grid;
+  return GF_OK;
+}
+
+__global__ void reset_counters_kernel(unsigned long long* c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
+
+// One L and one U sync-free sweep over several systems: each chunk list is
+// the systems' chunks merged level by level (see IluView).  counters[0..1]
+// are zeroed first by a kernel (no memset: the launches may be captured in
+// a graph), so the tickets start at 0 in every replay.
+template <typename T>
+int launch_batch_sweeps(hipStream_t s, const SweepSlot<T>* slots,
+                        const int2* lmerged, int nl, const int2* umerged,
+                        int nu, int grid, unsigned long long* counters,
+                        int* err) {
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters,
+                     2);
+  hipLaunchKernelGGL((trsv_syncfree_kernel<T, true, 1, true>), dim3(grid),
+                     dim3(kSweepThreads), 0, s, (const int4*)nullptr,
+                     (const int*)nullptr, (const int*)nullptr, nl,
+                     (const T*)nullptr, (const T*)nullptr, (T*)nullptr,
+                     (T*)nullptr, counters, 0ull, err, slots, lmerged);
+  hipLaunchKernelGGL((trsv_syncfree_kernel<T, false, 1, true>), dim3(grid),
+                     dim3(kSweepThreads), 0, s, (const int4*)nullptr,
+                     (const int*)nullptr, (const int*)nullptr, nu,
+                     (const T*)nullptr, (const T*)nullptr, (T*)nullptr,
+                     (T*)nullptr, counters + 1, 0ull, err, slots, umerged);
+  return hipGetLastError() == hipSuccess ? GF_OK : GF_ERR_DEVICE;
+}
+template int launch_batch_sweeps<double>(hipStream_t, const SweepSlot<double>*,
+                                         const int2*, int, const int2*, int,
+                                         int, unsigned long long*, int*);
+template int launch_batch_sweeps<float>(hipStream_t, const SweepSlot<float>*,
+                                        const int2*, int, const int2*, int,
+                                        int, unsigned long long*, int*);
 }  // namespace gf
 
 namespace {

@@ -1,6 +1,7 @@
 // GMRES driver interfaces (see krylov.hip).
 #pragma once
 #include <cstddef>
+#include <vector>
 
 #include "common.h"
 
@@ -68,5 +69,66 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
 // bounded dependency wait ran out) and the context an ILU handle launches on.
 int* ilu_sweep_err(gf_ilu h);
 gf_context ilu_context(gf_ilu h);
+
+// ---- batched sweeps (several systems in one launch per sweep)
+// The arrays of one system in a batched sweep launch: L sweep b -> tmp
+// (setting x to the sentinel row by row), U sweep tmp -> x.  tmp must hold
+// the sentinel before the L sweep (see fill in the batched GMRES step).
+template <typename T>
+struct SweepSlot {
+  const int4* lmeta;
+  const int4* umeta;
+  const int* lchunk;
+  const int* uchunk;
+  const int* cols;
+  const T* lu;
+  const T* b;
+  T* tmp;
+  T* x;
+  const int* go;  // batched launches sweep the system only while *go == 1
+};
+// The device schedule of an ILU analysis (chunk mode only) and the
+// dependency level of each chunk.
+struct IluView {
+  int n = 0;
+  const int4 *lmeta = nullptr, *umeta = nullptr;
+  const int *lchunk = nullptr, *uchunk = nullptr, *cols = nullptr;
+  const std::vector<int>* lchunk_level = nullptr;
+  const std::vector<int>* uchunk_level = nullptr;
+  int grid = 0;
+};
+int ilu_view(gf_ilu h, IluView* v);
+template <typename T>
+int launch_batch_sweeps(hipStream_t s, const SweepSlot<T>* slots,
+                        const int2* lmerged, int nl, const int2* umerged,
+                        int nu, int grid, unsigned long long* counters,
+                        int* err);
+// bit pattern the sweeps take for "not yet written"
+unsigned long long sweep_sentinel_bits(bool fp64);
+
+// ---- batched GMRES of the serving loop
+// Solves of several assembled and factored problems advance together: one
+// Arnoldi step of every solve is one fused launch sequence (gather and
+// precondition rhs, batched L and U sweeps, batched SpMV, batched DCGS2 dot
+// products, coefficients and basis update), and a fixed number of steps is
+// captured once as a HIP graph and replayed until the set of solves
+// changes.  Right-preconditioned restarted GMRES as gmres_solve (DCGS2,
+// cycle target 0.3 rtol, compensated FP64 residual at every cycle start),
+// with the preconditioner in FP64 (precision 0) or FP32 (precision 2).
+struct BatchJob {
+  gf_problem p = nullptr;      // assembled and factored
+  const double* loads = nullptr;  // host, 6 n, unpermuted
+  double* x = nullptr;            // host, 6 n, unpermuted
+  gf_solve_info info{};
+  int status = GF_OK;
+  bool done = false;
+};
+class BatchEngine;
+// device: the device of the problems; precision 0 or 2
+int batch_engine_create(int device, int precision, int max_slots,
+                        BatchEngine** out);
+void batch_engine_destroy(BatchEngine* e);
+// queues a job and waits until its solve has finished (thread-safe)
+int batch_engine_solve(BatchEngine* e, BatchJob* job);
 
 }  // namespace gf

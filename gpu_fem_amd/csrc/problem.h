@@ -6,6 +6,32 @@
 #include <vector>
 
 #include "common.h"
+#include "krylov.h"
+
+// A linear-static problem resident on the device (gf_problem_*): mesh,
+// patterns, maps, K and preconditioner values, Krylov work space.
+struct gf_problem_s {
+  gf_context ctx = nullptr;
+  gf_options opt{};
+  int nnodes = 0, nelems = 0, nbcs = 0;
+  int nnzb_k = 0, nnzb_m = 0;
+  std::vector<int> h_perm;
+  // device arrays
+  int *conn = nullptr, *bcs = nullptr, *perm = nullptr, *iperm = nullptr;
+  double *xpts = nullptr, *data = nullptr;
+  int *krowp = nullptr, *kcols = nullptr, *emap = nullptr;
+  int *trowp = nullptr, *tcols = nullptr, *tmap = nullptr;
+  double *kvals = nullptr, *mvals = nullptr;
+  float *kvals32 = nullptr, *mvals32 = nullptr;
+  double *rhs = nullptr, *sol = nullptr, *loads = nullptr;
+  double *work64 = nullptr, *work64b = nullptr;
+  float *t32a = nullptr, *t32b = nullptr, *t32c = nullptr;
+  gf_ilu ilu = nullptr;
+  gf_asm_plan plan = nullptr;  // element colouring for the atomic-free scatter
+  gf::KrylovWork kw;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double t_assemble = 0, t_factor = 0;
+};
 
 namespace gf {
 

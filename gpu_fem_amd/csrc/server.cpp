@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -19,6 +20,7 @@
 #include <vector>
 
 #include "common.h"
+#include "krylov.h"
 #include "problem.h"
 
 namespace {
@@ -64,6 +66,13 @@ struct gf_server_s {
   long long completed = 0, failed = 0, escalated = 0;
   std::vector<double> latencies;
   Clock::time_point t0 = Clock::now();
+  int class_bound[2] = {120, 360};  // sweep depth bounds of the size classes
+  // batched GMRES engines, one per preconditioner precision (FP64, FP32)
+  // and size class: the Krylov phase of concurrent requests of one class
+  // runs as fused launches; each engine has its own stream (null: each
+  // worker solves on its own stream)
+  static constexpr int kClasses = 3;
+  gf::BatchEngine* engine[3][kClasses] = {};
 };
 
 namespace {
@@ -166,7 +175,25 @@ void worker_main(gf_server_s* s) {
       if (st == GF_OK) st = gf_problem_factor(p);
       if (st == GF_OK) {
         res.x.resize(6 * (size_t)req.nnodes);
-        st = gf_problem_solve(p, req.loads.data(), res.x.data(), 0, &res.info);
+        // requests advance in lockstep inside an engine, so each engine
+        // takes requests of similar sweep depth (dependency levels)
+        int nl = 0, nu = 0;
+        gf_ilu_info(gf_problem_ilu(p), nullptr, &nl, &nu);
+        const int depth = std::max(nl, nu);
+        const int cls = depth < s->class_bound[0] ? 0
+                        : depth < s->class_bound[1] ? 1 : 2;
+        gf::BatchEngine* eng = s->engine[req.opt.precision][cls];
+        if (eng) {
+          gf::BatchJob job;
+          job.p = p;
+          job.loads = req.loads.data();
+          job.x = res.x.data();
+          st = gf::batch_engine_solve(eng, &job);
+          res.info = job.info;
+        } else {
+          st = gf_problem_solve(p, req.loads.data(), res.x.data(), 0,
+                                &res.info);
+        }
       }
       if (p) gf_problem_destroy(p);
       {
@@ -231,6 +258,27 @@ int gf_server_create(int device, double budget_gb, int slots,
   auto* s = new gf_server_s;
   s->device = device;
   s->budget = (size_t)(budget_gb * 1e9);
+  // GPUFEM_SERVE_BATCH=1: the Krylov phase of concurrent requests runs in
+  // the batched engines (fused launches, graph replays); by default each
+  // worker solves on its own stream, which measured faster on the served
+  // mix (the batched solves advance in lockstep, so a small request's step
+  // lasts as long as the largest request's step in its engine)
+  const char* be = getenv("GPUFEM_SERVE_BATCH");
+  if (be && atoi(be) == 1) {
+    int st = GF_OK;
+    for (int prec : {0, 2})
+      for (int c = 0; c < gf_server_s::kClasses && st == GF_OK; c++)
+        st = gf::batch_engine_create(device, prec, slots, &s->engine[prec][c]);
+    if (st != GF_OK) {
+      for (auto& row : s->engine)
+        for (auto*& e : row) {
+          gf::batch_engine_destroy(e);
+          e = nullptr;
+        }
+      delete s;
+      return st;
+    }
+  }
   for (int i = 0; i < slots; i++) s->workers.emplace_back(worker_main, s);
   *out = s;
   return GF_OK;
@@ -320,6 +368,8 @@ int gf_server_destroy(gf_server s) {
   }
   s->cv.notify_all();
   for (auto& t : s->workers) t.join();
+  for (auto& row : s->engine)
+    for (auto* e : row) gf::batch_engine_destroy(e);
   delete s;
   return GF_OK;
 }

@@ -1,5 +1,5 @@
 """Serving loop: requests of differing meshes admitted under an HBM budget
-and solved concurrently on worker streams.  Each result in the mix equals the
+and solved concurrently, on worker streams or in the batched GMRES engines.  Each result in the mix equals the
 same request solved alone (1e-10 relative; the only order-dependent step is
 the FP64 atomic accumulation in the Jacobian scatter), every result meets its
 tolerance, and admission respects the budget."""
@@ -37,8 +37,15 @@ def requests(seed=0, n=10):
 OPTS = dict(reorder=2, lev_fill=3, restart=80, rtol=1e-9, max_iters=20000)
 
 
-def test_mix_equals_alone(gpu_ctx):
-    print("[gpu] server mix vs alone, 10 requests", flush=True)
+# 0: each worker solves on its own stream; 1: the batched GMRES engines
+# (fused launches over the concurrent solves, replayed as a HIP graph)
+ENGINES = pytest.mark.parametrize("batch", [0, 1], ids=["streams", "batched"])
+
+
+@ENGINES
+def test_mix_equals_alone(gpu_ctx, batch, monkeypatch):
+    monkeypatch.setenv("GPUFEM_SERVE_BATCH", str(batch))
+    print(f"[gpu] server mix vs alone, 10 requests, batch={batch}", flush=True)
     reqs = requests()
     alone = []
     for m in reqs:
@@ -143,7 +150,8 @@ SERVED = dict(reorder=2, lev_fill=32, restart=200, rtol=1e-8, max_iters=20000,
               stall_cycles=4, precision=2)
 
 
-def test_served_mix_against_fp64_oracle(gpu_ctx):
+@ENGINES
+def test_served_mix_against_fp64_oracle(gpu_ctx, batch, monkeypatch):
     """Requests of the served mix arrive as a Poisson stream (rate 20/s) at
     a server with 4 slots; every served result has a true relative residual
     at or below 1e-8 evaluated by the oracle in compensated FP64, lies close
@@ -154,8 +162,10 @@ def test_served_mix_against_fp64_oracle(gpu_ctx):
     import oracle
     from gpu_helpers import oracle_problem
 
+    monkeypatch.setenv("GPUFEM_SERVE_BATCH", str(batch))
     reqs = served_mix_sample()
-    print(f"[gpu] served mix sample: {[m.name for m in reqs]}", flush=True)
+    print(f"[gpu] served mix sample (batch={batch}): {[m.name for m in reqs]}",
+          flush=True)
     rng = np.random.default_rng(11)
     arrivals = np.cumsum(rng.exponential(1.0 / 20.0, len(reqs)))
     srv = G.Server(budget_gb=200.0, slots=4)
