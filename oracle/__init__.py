@@ -44,12 +44,39 @@ def build(force: bool = False) -> None:
         subprocess.check_call(["make", "-s", "-C", _HERE, "ref"])
 
 
+def build_native(out_dir: str) -> str:
+    """Compile the oracle with -O3 -march=native for the host it runs on
+    (the CPU baseline's build; the committed build is portable) and return
+    the library's path."""
+    import subprocess
+
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "liboracle_native.so")
+    subprocess.check_call(["make", "-s", "-C", _HERE, "native",
+                           f"NATIVE_OUT={out}"])
+    return out
+
+
+def use_library(path: str) -> None:
+    """Route every oracle call through the library at ``path`` (same
+    interface; used by the CPU baseline for its native build)."""
+    global _LIB
+    _LIB = _load(path)
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
+        _LIB = _load(path)
+    return _LIB
+
+
+def _load(path):
+    """Load an oracle build and declare its functions."""
+    if True:
         L = ctypes.CDLL(path)
         L.orc_free.argtypes = [ctypes.c_void_p]
         L.orc_mitc4_element.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p,
@@ -100,8 +127,8 @@ def lib():
                                 ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 _f64p]
         L.orc_dense_solve.argtypes = [ctypes.c_int, _f64p, _f64p, _f64p]
-        _LIB = L
-    return _LIB
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+    return L
 
 
 def ref():

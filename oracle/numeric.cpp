@@ -28,6 +28,15 @@
 
 #include "oracle.h"
 
+// Host threads of the parallel loops (element matrices, SpMV, residual,
+// Gram-Schmidt vector operations); 1 by default.  Results do not depend on
+// the count: element matrices are summed in element order, and reductions
+// use a fixed split into kRedChunks partial sums when more than one thread
+// runs.  The ILU factorisation and the triangular solves stay sequential.
+static int g_threads = 1;
+constexpr int kRedChunks = 64;
+extern "C" void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
 extern "C" void orc_assemble_jacobian(int nelems, const int* conn,
                                       const double* xpts, const double* vars,
                                       const double* data,
@@ -38,6 +47,44 @@ extern "C" void orc_assemble_jacobian(int nelems, const int* conn,
   // threads), then summed in element order so the result does not depend on
   // the thread count.
   (void)nthreads;
+  if (g_threads > 1) {
+    // element matrices of a chunk of elements in parallel, then summed in
+    // element order
+    constexpr int kChunk = 4096;
+    std::vector<double> Kc((size_t)kChunk * 576), Rc((size_t)kChunk * 24);
+    for (int e0 = 0; e0 < nelems; e0 += kChunk) {
+      const int ne = std::min(kChunk, nelems - e0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+      for (int q = 0; q < ne; q++) {
+        const int e = e0 + q;
+        double xe[12], ve[24];
+        for (int a = 0; a < 4; a++) {
+          int nd = conn[4 * e + a];
+          for (int k = 0; k < 3; k++) xe[3 * a + k] = xpts[3 * nd + k];
+          for (int k = 0; k < 6; k++) ve[6 * a + k] = vars ? vars[6 * nd + k] : 0.0;
+        }
+        orc_mitc4_element(xe, ve, &data[4 * e], &Kc[576 * (size_t)q],
+                          &Rc[24 * (size_t)q], nullptr);
+      }
+      for (int q = 0; q < ne; q++) {
+        const int e = e0 + q;
+        const double* Ke = &Kc[576 * (size_t)q];
+        for (int a = 0; a < 4; a++)
+          for (int b = 0; b < 4; b++) {
+            double* blk = &kvals[36 * (size_t)elem_ind_map[16 * e + 4 * a + b]];
+            for (int r = 0; r < 6; r++)
+              for (int c = 0; c < 6; c++)
+                blk[6 * r + c] += Ke[24 * (6 * a + r) + 6 * b + c];
+          }
+        if (res)
+          for (int a = 0; a < 4; a++) {
+            int nd = perm ? perm[conn[4 * e + a]] : conn[4 * e + a];
+            for (int k = 0; k < 6; k++) res[6 * nd + k] += Rc[24 * (size_t)q + 6 * a + k];
+          }
+      }
+    }
+    return;
+  }
   std::vector<double> Ke(576), Re(24);
   for (int e = 0; e < nelems; e++) {
     double xe[12], ve[24];
@@ -243,6 +290,7 @@ void bsr_trsv(int n, const int* rowp, const int* cols, const T* lu, const T* b,
 template <typename T>
 void bsr_spmv(int n, const int* rowp, const int* cols, const T* vals,
               const T* x, T* y) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
   for (int i = 0; i < n; i++) {
     T s[6] = {0, 0, 0, 0, 0, 0};
     for (int k = rowp[i]; k < rowp[i + 1]; k++) {
@@ -262,6 +310,7 @@ void bsr_spmv(int n, const int* rowp, const int* cols, const T* vals,
 void bsr_resid_comp(int n, const int* rowp, const int* cols,
                     const double* vals, const double* x, const double* b,
                     double* r) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
   for (int i = 0; i < n; i++) {
     double s[6], c[6];
     for (int q = 0; q < 6; q++) {
@@ -286,10 +335,30 @@ void bsr_resid_comp(int n, const int* rowp, const int* cols,
   }
 }
 
-double nrm2(const std::vector<double>& v) {
+// sum_t a[t] b[t]: serial with one thread, else kRedChunks partial sums in
+// parallel added in chunk order
+double pdot(const double* a, const double* b, size_t N) {
+  if (g_threads <= 1) {
+    double s = 0;
+    for (size_t t = 0; t < N; t++) s += a[t] * b[t];
+    return s;
+  }
+  double part[kRedChunks];
+  const size_t len = (N + kRedChunks - 1) / kRedChunks;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int c = 0; c < kRedChunks; c++) {
+    double s = 0;
+    const size_t t1 = std::min(N, (c + 1) * len);
+    for (size_t t = c * len; t < t1; t++) s += a[t] * b[t];
+    part[c] = s;
+  }
   double s = 0;
-  for (double a : v) s += a * a;
-  return std::sqrt(s);
+  for (int c = 0; c < kRedChunks; c++) s += part[c];
+  return s;
+}
+
+double nrm2(const std::vector<double>& v) {
+  return std::sqrt(pdot(v.data(), v.data(), v.size()));
 }
 
 }  // namespace
@@ -408,14 +477,17 @@ extern "C" int orc_gmres(int n, const int* krowp, const int* kcols,
       inner_op(z.data(), w.data());
       total_it++;
       for (int i = 0; i <= j; i++) {  // modified Gram-Schmidt
-        double h = 0;
-        for (size_t t = 0; t < N; t++) h += w[t] * V[i][t];
+        const double h = pdot(w.data(), V[i].data(), N);
         H[i * m + j] = h;
-        for (size_t t = 0; t < N; t++) w[t] -= h * V[i][t];
+        const double* vi = V[i].data();
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
+        for (size_t t = 0; t < N; t++) w[t] -= h * vi[t];
       }
       double hn = nrm2(w);
       H[(j + 1) * m + j] = hn;
-      for (size_t t = 0; t < N; t++) V[j + 1][t] = hn > 0 ? w[t] / hn : 0.0;
+      double* vn = V[j + 1].data();
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
+      for (size_t t = 0; t < N; t++) vn[t] = hn > 0 ? w[t] / hn : 0.0;
       for (int i = 0; i < j; i++) {
         double a = H[i * m + j], c = H[(i + 1) * m + j];
         H[i * m + j] = cs[i] * a + sn[i] * c;

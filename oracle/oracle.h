@@ -106,6 +106,8 @@ int orc_gmres(int nnodes, const int* krowp, const int* kcols,
               int restart, int max_iters, double rtol, int mixed,
               double* stats);
 int orc_dense_solve(int n, const double* A, const double* b, double* x);
+/* host threads of the parallel loops (default 1; results independent of it) */
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }
