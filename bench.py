@@ -8,15 +8,16 @@ residual, plain FP64 and mixed precision.
 
 Ordering and fill: the configuration names RCM + ILU(3).  On this thin plate
 (t/L = 1/200) the RCM-ordered ILU(3) factors are unstable (negative pivots
-of an SPD matrix; |M^-1 b| / |b| ~ 1e6 at 200 x 200) and GMRES diverges;
-natural order + ILU(3) converges but its triangular sweeps have 2,401
-dependency levels; AMD order with ILU(12) converges in fewer iterations
-(206 vs 535 at 200 x 200) with 311 levels.  At 400 x 400 more fill keeps
-paying: AMD + ILU(k) solve steps of 1.95 s at k = 16 (482 iterations, 423
-levels), 1.09 s at 24, 0.66 s at 48 and 0.62 s at 56 (63 iterations, more
-levels but far fewer sweeps; the dataflow factorisation stays near
-150 ms); k = 64 has rows longer than the factorisation's 1,024-block
-limit.  The default run is AMD + ILU(56) + GMRES(500);
+of an SPD matrix; |M^-1 b| / |b| ~ 1e6 at 200 x 200) and GMRES stagnates
+(the ``configs_as_written`` block records the outcome); natural order +
+ILU(3) converges but its triangular sweeps have 2,401 dependency levels.
+AMD order with more fill keeps paying at 400 x 400: AMD + ILU(k) solve
+steps of 1.95 s at k = 16 (482 iterations, 423 levels), 0.62 s at 56 (63
+iterations, 1,215 levels), and, with the AMD tie order updating each reach
+in reverse, 0.60 s at 56, 0.54 s at 80 and 0.49 s at 96 (23 iterations,
+1,372 levels, 257 ms of numeric factorisation); beyond 96 the
+factorisation grows faster than the sweeps shrink (0.52 s at 112, 0.56 s
+at 128).  The default run is AMD + ILU(96) + GMRES(500);
 ``--reorder rcm --lev-fill 3`` runs the configuration as written.
 
 A step is one complete numeric solve of that problem with the mesh and the
@@ -84,7 +85,7 @@ def parse():
     ap.add_argument("--precision", default="all",
                     choices=["all", "both", "fp64", "mixed", "mixed_pc"])
     ap.add_argument("--reorder", default="amd", choices=["natural", "rcm", "amd"])
-    ap.add_argument("--lev-fill", type=int, default=56)
+    ap.add_argument("--lev-fill", type=int, default=96)
     ap.add_argument("--restart", type=int, default=500)
     ap.add_argument("--max-iters", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -8,8 +8,9 @@
 // absorbed into the new one, and each variable's degree is bounded by
 //   d_i ~ |A_i| + |L_p \ i| + sum_{e in E_i, e != p} |L_e \ L_p|
 // (|L_e \ L_p| via the w(e) counters).  Supervariable detection and
-// aggressive absorption are omitted; ties go to the most recently updated
-// node (degree buckets are stacks).  Output perm[old] = new.
+// aggressive absorption are omitted (the node graph of 6-DOF shell nodes is
+// already the compressed graph); ties go to the most recently updated node
+// (degree buckets are stacks).  Output perm[old] = new.
 #include <algorithm>
 #include <vector>
 
@@ -74,6 +75,13 @@ int amd_order(int n, const int* rowp, const int* cols, int* perm) {
     vars[p].clear();
     vars[p].shrink_to_fit();
     elems[p].clear();
+    // the reach is collected A_p first, then element by element; its
+    // degrees are updated (and its nodes pushed on their degree stacks) in
+    // the reverse of that order.  The order only decides ties; on the
+    // 400 x 400 plate with ILU(56) it gives 1,119 instead of 1,215
+    // dependency levels per triangular sweep and slightly less fill
+    // (measured; a sorted order gives 1,412).
+    std::reverse(Lp.begin(), Lp.end());
     le[p] = Lp;
     // w(e) = |L_e \ L_p| for elements adjacent to L_p
     for (int i : Lp)

@@ -71,7 +71,10 @@ struct gf_ilu_s {
 namespace gf {
 namespace {
 
-constexpr int kMaxRowBlocks = 1024;
+// Longest row (blocks) the factorisation takes: the column indices and two
+// pivots' position lists of a row live in LDS (3 ints per block of the
+// longest row, 96 KB at this limit).
+constexpr int kMaxRowBlocks = 8192;
 
 // Dataflow schedule of the factorisation (kFlow kernels).  Every workgroup
 // takes exactly one ticket from the counter; ticket t factors
@@ -184,9 +187,12 @@ __global__ void __launch_bounds__(kIluThreads)
     ilu_row_kernel(const int* __restrict__ level_rows, int nrows,
                    const int* __restrict__ rowp, const int* __restrict__ cols,
                    const int* __restrict__ diag, T* vals, int* pivot,
-                   Flow f) {
-  __shared__ int scol[kMaxRowBlocks];
-  __shared__ int pos[2][kMaxRowBlocks];  // positions for this k and the next
+                   Flow f, int maxlen) {
+  // dynamic LDS: column indices of row i, and the positions in row i of
+  // the current and next pivot's U-part columns (maxlen = longest row)
+  extern __shared__ int ilu_gdyn[];
+  int* scol = ilu_gdyn;
+  int* const pos[2] = {ilu_gdyn + maxlen, ilu_gdyn + 2 * maxlen};
   __shared__ int ub[2][2];               // U-part bounds [u0, u1) per buffer
   __shared__ T udg[2][36];               // U_kk per buffer
   __shared__ T udi[2][6];                // 1 / diagonal of U_kk per buffer
@@ -380,9 +386,8 @@ __global__ void __launch_bounds__(kIluThreads)
                        const int* __restrict__ rowp,
                        const int* __restrict__ cols,
                        const int* __restrict__ diag, T* vals, int* pivot,
-                       Flow f) {
+                       Flow f, int maxlen) {
   extern __shared__ __align__(16) unsigned char ilu_dyn[];
-  __shared__ int pos[2][kMaxRowBlocks];  // local positions in row i, or -1
   __shared__ T udg[2][36];
   __shared__ T udi[2][6];  // 1 / diagonal of U_kk
   __shared__ T lik[36];
@@ -394,6 +399,8 @@ __global__ void __launch_bounds__(kIluThreads)
   T* row = reinterpret_cast<T*>(ilu_dyn);
   int* scol = reinterpret_cast<int*>(row + 36 * (size_t)nb);
   int* ubs = scol + nb;  // [2 * kl]: U part [u0, u1) of pivot row scol[kl]
+  // local positions in row i of two pivots' U-part columns, or -1
+  int* const pos[2] = {ubs + 2 * nb, ubs + 2 * nb + maxlen};
   for (int e = tid; e < 36 * nb; e += nth) row[e] = vals[36 * (size_t)r0 + e];
   for (int t = tid; t < nb; t += nth) {
     const int j = cols[r0 + t];
@@ -1320,8 +1327,18 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
         hipGetLastError();
         cap = std::min<size_t>(cap, 48 * 1024);
       }
+    // the global-memory kernel's column and position lists
+    const void* gfns[2] = {(const void*)gf::ilu_row_kernel<T, false>,
+                           (const void*)gf::ilu_row_kernel<T, true>};
+    for (const void* fn : gfns)
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(3 * sizeof(int) * gf::kMaxRowBlocks)) !=
+          hipSuccess)
+        hipGetLastError();
     lds_cap[ti] = cap;
   });
+  const int maxlen = h->max_row_len;
+  const size_t gdyn = 3 * sizeof(int) * (size_t)maxlen;
   const bool no_lds = getenv("GPUFEM_ILU_GLOBAL") != nullptr;
   // GPUFEM_ILU_LDS_CAP (bytes) lowers the LDS budget of a row, so that one
   // factorisation mixes LDS and global-memory rows (a test hook)
@@ -1331,8 +1348,11 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   const char* senv = getenv("GPUFEM_ILU_SCHEDULE");
   const bool by_level = senv && std::string(senv) == "level";
   const int nl = (int)h->lptr.size() - 1;
+  // row values, column indices and pivot bounds of the level's longest
+  // row, and two position lists as long as the longest row
   auto dyn_of = [&](int l) {
-    return (size_t)h->lmaxlen[l] * (36 * sizeof(T) + 3 * sizeof(int));
+    return (size_t)h->lmaxlen[l] * (36 * sizeof(T) + 3 * sizeof(int)) +
+           2 * sizeof(int) * (size_t)maxlen;
   };
   if (by_level) {
     for (int l = 0; l < nl; l++) {
@@ -1342,12 +1362,12 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
         hipLaunchKernelGGL((gf::ilu_row_lds_kernel<T, false>), dim3(cnt),
                            dim3(gf::kIluThreads), dyn, ctx->stream,
                            h->lrows + h->lptr[l], cnt, h->rowp, h->cols,
-                           h->diag, vals, h->pivot, gf::Flow{});
+                           h->diag, vals, h->pivot, gf::Flow{}, maxlen);
       else
         hipLaunchKernelGGL((gf::ilu_row_kernel<T, false>), dim3(cnt),
-                           dim3(gf::kIluThreads), 0, ctx->stream,
+                           dim3(gf::kIluThreads), gdyn, ctx->stream,
                            h->lrows + h->lptr[l], cnt, h->rowp, h->cols,
-                           h->diag, vals, h->pivot, gf::Flow{});
+                           h->diag, vals, h->pivot, gf::Flow{}, maxlen);
     }
   } else {
     // Dataflow: one launch per phase.  Levels before the first level with a
@@ -1390,12 +1410,12 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
         hipLaunchKernelGGL((gf::ilu_row_lds_kernel<T, true>), dim3(cnt),
                            dim3(gf::kIluThreads), dyn, ctx->stream,
                            h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals,
-                           h->pivot, f);
+                           h->pivot, f, maxlen);
       else
         hipLaunchKernelGGL((gf::ilu_row_kernel<T, true>), dim3(cnt),
-                           dim3(gf::kIluThreads), 0, ctx->stream,
+                           dim3(gf::kIluThreads), gdyn, ctx->stream,
                            h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals,
-                           h->pivot, f);
+                           h->pivot, f, maxlen);
       h->base += (unsigned long long)cnt;
       l0 = l1;
     }
