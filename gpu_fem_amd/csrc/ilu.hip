@@ -365,6 +365,254 @@ __global__ void __launch_bounds__(kIluThreads)
   if (kFlow) flow_publish(i, f);
 }
 
+// Rows too long for LDS, with the pivots applied in batches.  The values of
+// row i stay in global memory.  A batch takes up to nbatch consecutive
+// pivots whose rows are final; wavefront 0 forms their L_ik in order, each
+// applied at once to the batch's own later L-part blocks (held in LDS), and
+// then one pass reads every (block, column) pair of the rest of the row that
+// a pivot of the batch updates once, subtracts the batch's contributions
+// L_ik U_kj in pivot order and writes it once: one read-modify-write of the
+// touched part of the row per batch instead of one per pivot, and each U_kj
+// column loaded once for its six rows.  The pass walks the pivots' U-part
+// lists (a pair is taken by the first pivot of the batch that updates it),
+// so a batch of one pivot costs what a pivot step of ilu_row_kernel does.
+// Every entry sees the operations of ilu_row_kernel in the same order, so
+// the factors agree bit for bit.
+// Dynamic LDS: the column indices of row i; per pivot of the batch, the
+// position in row i of each of its U-part columns (or -1) and, per position
+// of row i, 1 + the index in the pivot's U part (0: not updated).
+#ifndef GPUFEM_ILU_BATCH
+#define GPUFEM_ILU_BATCH 8
+#endif
+constexpr int kIluBatch = GPUFEM_ILU_BATCH;
+constexpr int kIluPairSlots = kIluBatch * (kIluBatch - 1) / 2;
+// slot of the pivot pair (p, q), p < q < kIluBatch
+__device__ __forceinline__ int batch_pair(int p, int q) {
+  return p * (2 * kIluBatch - p - 1) / 2 + (q - p - 1);
+}
+// dynamic LDS bytes per position of the longest row, batch of nbatch
+__host__ __device__ constexpr size_t batch_lds_per_pos(int nbatch) {
+  return sizeof(int) + 2 * sizeof(short) * (size_t)nbatch;
+}
+template <typename T, bool kFlow>
+__global__ void __launch_bounds__(kIluThreads)
+    ilu_row_batch_kernel(const int* __restrict__ level_rows, int nrows,
+                         const int* __restrict__ rowp,
+                         const int* __restrict__ cols,
+                         const int* __restrict__ diag, T* vals, int* pivot,
+                         Flow f, int nbatch, int maxlen) {
+  extern __shared__ __align__(16) int ilu_bdyn[];
+  __shared__ int s_u0[kIluBatch], s_nu[kIluBatch];
+  __shared__ T s_ukk[kIluBatch][36];
+  __shared__ T s_udi[kIluBatch][6];  // 1 / diagonal of U_kk
+  __shared__ T s_lik[kIluBatch][36];
+  __shared__ T s_ab[kIluBatch][36];       // A_ik of the batch's pivots
+  __shared__ T s_uq[kIluPairSlots][36];   // U_kp,kq, p < q in the batch
+  __shared__ int s_row, s_rdy[2];
+  const int i = flow_row<kFlow>(level_rows, nrows, f, &s_row);
+  if (i < 0) return;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int r0 = rowp[i], r1 = rowp[i + 1], nb = r1 - r0, dl = diag[i] - r0;
+  int* scol = ilu_bdyn;
+  // hit[p * maxlen + pl] = 1 + index in U(k_p) of position pl, or 0
+  unsigned short* hit = reinterpret_cast<unsigned short*>(ilu_bdyn + maxlen);
+  // upos[p * maxlen + t] = position in row i of U(k_p)'s column t, or -1
+  short* upos = reinterpret_cast<short*>(hit + (size_t)nbatch * maxlen);
+  for (int t = tid; t < nb; t += nth) scol[t] = cols[r0 + t];
+  for (int e = tid; e < nbatch * maxlen; e += nth) hit[e] = 0;
+  if (tid == 0) s_rdy[0] = 0;
+  int par = 0;
+  __syncthreads();
+  // position of column j among scol[lo..nb), or -1
+  auto find = [&](int j, int lo) {
+    int hi = nb;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (scol[mid] < j)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return (lo < nb && scol[lo] == j) ? lo : -1;
+  };
+  for (int kb = 0; kb < dl;) {
+    int np = min(nbatch, dl - kb);
+    if (kFlow) {
+      // the batch: the pivots from kb on whose rows are known to be final
+      flow_wait(scol, dl, kb, s_rdy, par, f);
+      np = min(np, s_rdy[par] - kb);
+    }
+    if (tid < np) {
+      const int k = scol[kb + tid];
+      const int u0 = diag[k] + 1;
+      s_u0[tid] = u0;
+      s_nu[tid] = rowp[k + 1] - u0;
+    }
+    for (int e = tid; e < 36 * np; e += nth) {
+      const int p = e / 36, q = e % 36;
+      const T u = vals[36 * (size_t)diag[scol[kb + p]] + q];
+      s_ukk[p][q] = u;
+      if (q % 7 == 0) s_udi[p][q / 7] = T(1) / u;
+      s_ab[p][q] = vals[36 * (size_t)(r0 + kb + p) + q];
+    }
+    __syncthreads();
+    int tot = 0;  // U-part columns of the batch's pivots
+    for (int p = 0; p < np; p++) {
+      const int u0 = s_u0[p], nu = s_nu[p];
+      for (int t = tid; t < nu; t += nth) {
+        const int pl = find(cols[u0 + t], kb + p + 1);
+        upos[p * maxlen + t] = (short)pl;
+        if (pl >= 0) hit[p * maxlen + pl] = (unsigned short)(t + 1);
+      }
+      tot += nu;
+    }
+    __syncthreads();
+    for (int e = tid; e < 36 * np * np; e += nth) {
+      const int pq = e / 36, m = e % 36, p = pq / np, q = pq % np;
+      if (q <= p) continue;
+      const int v = hit[p * maxlen + kb + q];
+      if (v)
+        s_uq[batch_pair(p, q)][m] =
+            vals[36 * (size_t)(s_u0[p] + v - 1) + m];
+    }
+    __syncthreads();
+    if (tid < 64) {
+      // wavefront 0: L_ik of each pivot, each applied to the batch's later
+      // L-part blocks before the next is formed
+      for (int p = 0; p < np; p++) {
+        if (tid < 6) {
+          const int r = tid;
+          T x[6];
+#pragma unroll
+          for (int c = 0; c < 6; c++) {
+            T v = s_ab[p][6 * r + c];
+#pragma unroll
+            for (int m = 0; m < c; m++) v -= x[m] * s_ukk[p][6 * m + c];
+            x[c] = v * s_udi[p][c];
+          }
+#pragma unroll
+          for (int c = 0; c < 6; c++) {
+            s_ab[p][6 * r + c] = x[c];
+            s_lik[p][6 * r + c] = x[c];
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (tid < 36) {
+          const int r = tid / 6, c = tid % 6;
+          for (int q = p + 1; q < np; q++) {
+            if (!hit[p * maxlen + kb + q]) continue;
+            const T* U = s_uq[batch_pair(p, q)];
+            T acc = 0;
+#pragma unroll
+            for (int m = 0; m < 6; m++) acc += s_lik[p][6 * r + m] * U[6 * m + c];
+            s_ab[q][6 * r + c] = s_ab[q][6 * r + c] - acc;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < 36 * np; e += nth)
+      vals[36 * (size_t)(r0 + kb + e / 36) + e % 36] = s_ab[e / 36][e % 36];
+    // the rest of the row: (block, column) pairs of the pivots' U-part
+    // lists, each taken by the first pivot of the batch that updates it
+    const int pl0 = kb + np;
+    for (int w = tid; w < 6 * tot; w += nth) {
+      // L_ik is re-read from LDS per pair (hoisted out of the loop it would
+      // take 36 registers per pivot)
+      __asm__ volatile("" ::: "memory");
+      int p = 0, rem = w;
+      while (rem >= 6 * s_nu[p]) rem -= 6 * s_nu[p++];
+      const int t = rem / 6, c = rem % 6;
+      const int pl = upos[p * maxlen + t];
+      if (pl < pl0) continue;  // not in the row (-1), or in the batch's L part
+      bool first = true;
+      for (int q = 0; q < p; q++) first = first && !hit[q * maxlen + pl];
+      if (!first) continue;
+      int tt[kIluBatch];
+#pragma unroll
+      for (int q = 0; q < kIluBatch; q++)
+        tt[q] = (q >= p && q < np) ? (int)hit[q * maxlen + pl] - 1 : -1;
+      T* A = vals + 36 * (size_t)(r0 + pl) + c;
+      T a[6];
+#pragma unroll
+      for (int r = 0; r < 6; r++) a[r] = A[6 * r];
+#pragma unroll
+      for (int q0 = 0; q0 < kIluBatch; q0 += 4) {
+        T u[4][6];
+#pragma unroll
+        for (int qq = 0; qq < 4 && q0 + qq < kIluBatch; qq++) {
+          const int q = q0 + qq;
+          const T* U =
+              vals + 36 * (size_t)(tt[q] >= 0 ? s_u0[q] + tt[q] : 0) + c;
+#pragma unroll
+          for (int m = 0; m < 6; m++) u[qq][m] = tt[q] >= 0 ? U[6 * m] : T(0);
+        }
+#pragma unroll
+        for (int qq = 0; qq < 4 && q0 + qq < kIluBatch; qq++) {
+          const int q = q0 + qq;
+          if (tt[q] < 0) continue;
+#pragma unroll
+          for (int r = 0; r < 6; r++) {
+            T v = 0;
+#pragma unroll
+            for (int m = 0; m < 6; m++) v += s_lik[q][6 * r + m] * u[qq][m];
+            a[r] = a[r] - v;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 6; r++) A[6 * r] = a[r];
+    }
+    __syncthreads();
+    // clear the batch's hits
+    for (int p = 0; p < np; p++)
+      for (int t = tid; t < s_nu[p]; t += nth) {
+        const int pl = upos[p * maxlen + t];
+        if (pl >= 0) hit[p * maxlen + pl] = 0;
+      }
+    __syncthreads();
+    kb += np;
+  }
+  __syncthreads();
+  // factor the diagonal block (no pivoting)
+  T* D = vals + 36 * (size_t)(r0 + dl);
+  if (tid == 0) {
+    for (int c = 0; c < 6; c++) {
+      const T piv = D[6 * c + c];
+      if (piv == T(0)) {
+        atomicMin(pivot, i);
+        break;
+      }
+      for (int r = c + 1; r < 6; r++) {
+        const T l = D[6 * r + c] / piv;
+        D[6 * r + c] = l;
+        for (int m = c + 1; m < 6; m++) D[6 * r + m] -= l * D[6 * c + m];
+      }
+    }
+  }
+  __syncthreads();
+  // U_ij = L_ii^-1 A_ij for the blocks right of the diagonal
+  const int nr = nb - (dl + 1);
+  for (int w = tid; w < 6 * nr; w += nth) {
+    const int t = w / 6, c = w % 6;
+    T* A = D + 36 * (size_t)(1 + t);
+    T x[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      T s = A[6 * r + c];
+#pragma unroll
+      for (int m = 0; m < r; m++) s -= D[6 * r + m] * x[m];
+      x[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; r++) A[6 * r + c] = x[r];
+  }
+  if (kFlow) flow_publish(i, f);
+}
+
 // The same factorisation with row i held in LDS for the whole row: L_ik is
 // formed from, and the updates A_ij -= L_ik U_kj are applied to, the LDS
 // copy, so the only global accesses inside the pivot loop are reads of the
@@ -1306,6 +1554,7 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   // rows whose values fit in LDS (dynamic LDS per level: its longest row)
   // are factored there; the other levels run ilu_row_kernel
   static size_t lds_cap[2] = {0, 0};
+  static size_t batch_cap[2] = {0, 0};
   static std::once_flag once[2];
   const int ti = sizeof(T) == 8 ? 0 : 1;
   std::call_once(once[ti], [&] {
@@ -1327,7 +1576,7 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
         hipGetLastError();
         cap = std::min<size_t>(cap, 48 * 1024);
       }
-    // the global-memory kernel's column and position lists
+    // the global-memory kernels' column and position lists
     const void* gfns[2] = {(const void*)gf::ilu_row_kernel<T, false>,
                            (const void*)gf::ilu_row_kernel<T, true>};
     for (const void* fn : gfns)
@@ -1335,11 +1584,73 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
                               (int)(3 * sizeof(int) * gf::kMaxRowBlocks)) !=
           hipSuccess)
         hipGetLastError();
+    const void* bfns[2] = {(const void*)gf::ilu_row_batch_kernel<T, false>,
+                           (const void*)gf::ilu_row_batch_kernel<T, true>};
+    size_t bcap = total;
+    for (const void* fn : bfns) {
+      hipFuncAttributes fa{};
+      if (hipFuncGetAttributes(&fa, fn) != hipSuccess ||
+          total <= fa.sharedSizeBytes) {
+        bcap = 0;
+        break;
+      }
+      bcap = std::min(bcap, total - fa.sharedSizeBytes);
+    }
+    for (const void* fn : bfns)
+      if (bcap > 0 &&
+          hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bcap) != hipSuccess) {
+        hipGetLastError();
+        bcap = 0;
+      }
+    batch_cap[ti] = bcap;
     lds_cap[ti] = cap;
   });
   const int maxlen = h->max_row_len;
   const size_t gdyn = 3 * sizeof(int) * (size_t)maxlen;
   const bool no_lds = getenv("GPUFEM_ILU_GLOBAL") != nullptr;
+  // rows in global memory: pivots applied in batches (ilu_row_batch_kernel)
+  // of as many pivots as the LDS holds position lists for (at most
+  // kIluBatch); GPUFEM_ILU_BATCH_N lowers the batch (a test hook), and
+  // GPUFEM_ILU_GLOBAL_KERNEL=pivot selects ilu_row_kernel (one pass over the
+  // row per pivot)
+  int nbatch = 0;
+  {
+    // two workgroups per CU where the LDS allows
+    const size_t cap2 = std::min(batch_cap[ti], (size_t)64 * 1024);
+    for (int nbt = gf::kIluBatch; nbt >= 1 && !nbatch; nbt--)
+      if (gf::batch_lds_per_pos(nbt) * maxlen <= cap2) nbatch = nbt;
+    for (int nbt = gf::kIluBatch; nbt >= 1 && !nbatch; nbt--)
+      if (gf::batch_lds_per_pos(nbt) * maxlen <= batch_cap[ti]) nbatch = nbt;
+    if (const char* benv = getenv("GPUFEM_ILU_BATCH_N"))
+      nbatch = std::min(nbatch, std::max(1, atoi(benv)));
+    const char* kenv = getenv("GPUFEM_ILU_GLOBAL_KERNEL");
+    if (kenv && std::string(kenv) == "pivot") nbatch = 0;
+  }
+  const size_t bdyn = gf::batch_lds_per_pos(nbatch) * (size_t)maxlen;
+  auto launch_global = [&](bool flow, const int* rows, int cnt,
+                           const gf::Flow& f) {
+    if (nbatch > 0) {
+      if (flow)
+        hipLaunchKernelGGL((gf::ilu_row_batch_kernel<T, true>), dim3(cnt),
+                           dim3(gf::kIluThreads), bdyn, ctx->stream, rows,
+                           cnt, h->rowp, h->cols, h->diag, vals, h->pivot, f,
+                           nbatch, maxlen);
+      else
+        hipLaunchKernelGGL((gf::ilu_row_batch_kernel<T, false>), dim3(cnt),
+                           dim3(gf::kIluThreads), bdyn, ctx->stream, rows,
+                           cnt, h->rowp, h->cols, h->diag, vals, h->pivot, f,
+                           nbatch, maxlen);
+    } else if (flow) {
+      hipLaunchKernelGGL((gf::ilu_row_kernel<T, true>), dim3(cnt),
+                         dim3(gf::kIluThreads), gdyn, ctx->stream, rows, cnt,
+                         h->rowp, h->cols, h->diag, vals, h->pivot, f, maxlen);
+    } else {
+      hipLaunchKernelGGL((gf::ilu_row_kernel<T, false>), dim3(cnt),
+                         dim3(gf::kIluThreads), gdyn, ctx->stream, rows, cnt,
+                         h->rowp, h->cols, h->diag, vals, h->pivot, f, maxlen);
+    }
+  };
   // GPUFEM_ILU_LDS_CAP (bytes) lowers the LDS budget of a row, so that one
   // factorisation mixes LDS and global-memory rows (a test hook)
   size_t cap = lds_cap[ti];
@@ -1364,10 +1675,7 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
                            h->lrows + h->lptr[l], cnt, h->rowp, h->cols,
                            h->diag, vals, h->pivot, gf::Flow{}, maxlen);
       else
-        hipLaunchKernelGGL((gf::ilu_row_kernel<T, false>), dim3(cnt),
-                           dim3(gf::kIluThreads), gdyn, ctx->stream,
-                           h->lrows + h->lptr[l], cnt, h->rowp, h->cols,
-                           h->diag, vals, h->pivot, gf::Flow{}, maxlen);
+        launch_global(false, h->lrows + h->lptr[l], cnt, gf::Flow{});
     }
   } else {
     // Dataflow: one launch per phase.  Levels before the first level with a
@@ -1412,10 +1720,7 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
                            h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals,
                            h->pivot, f, maxlen);
       else
-        hipLaunchKernelGGL((gf::ilu_row_kernel<T, true>), dim3(cnt),
-                           dim3(gf::kIluThreads), gdyn, ctx->stream,
-                           h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals,
-                           h->pivot, f, maxlen);
+        launch_global(true, h->lrows + r0, cnt, f);
       h->base += (unsigned long long)cnt;
       l0 = l1;
     }

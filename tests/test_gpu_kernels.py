@@ -264,6 +264,40 @@ def test_ilu_lds_rows_match_global_rows(gpu_ctx, reorder, k, monkeypatch):
     ilu.close()
 
 
+@pytest.mark.parametrize("nbatch", [1, 3, 8])
+def test_ilu_batched_pivots_match_per_pivot_rows(gpu_ctx, nbatch, monkeypatch):
+    """Rows in global memory with the pivots applied in batches
+    (ilu_row_batch_kernel) and one pivot at a time (ilu_row_kernel): every
+    entry sees the same operations in the same order, so the factors agree
+    bit for bit, FP64 and FP32, on both schedules.  Complete LU of a small
+    wingbox: rows of up to 250 blocks, up to ~200 pivots a row."""
+    say(f"[gpu] ilu batched vs per-pivot global rows, batch {nbatch}")
+    m = M.wingbox(nc=24, nh=4, nsb=4, nribs=8, seed=5)
+    s = oracle_problem(m, "amd", 1 << 29)
+    monkeypatch.setenv("GPUFEM_ILU_GLOBAL", "1")
+    ilu = G.Ilu(gpu_ctx, s["mrp"], s["mcp"])
+    for dt in (np.float64, np.float32):
+        monkeypatch.setenv("GPUFEM_ILU_GLOBAL_KERNEL", "pivot")
+        d_ref = dev(s["mv"].astype(dt))
+        ilu.factor(d_ref)
+        gpu_ctx.sync()
+        ref = host(d_ref)
+        monkeypatch.delenv("GPUFEM_ILU_GLOBAL_KERNEL")
+        monkeypatch.setenv("GPUFEM_ILU_BATCH_N", str(nbatch))
+        for sched in ("flow", "level"):
+            if sched == "level":
+                monkeypatch.setenv("GPUFEM_ILU_SCHEDULE", "level")
+            d_a = dev(s["mv"].astype(dt))
+            ilu.factor(d_a)
+            gpu_ctx.sync()
+            a = host(d_a)
+            assert np.array_equal(a.view(np.uint8), ref.view(np.uint8)), \
+                f"{np.dtype(dt).name} {sched} batch {nbatch}"
+        monkeypatch.delenv("GPUFEM_ILU_SCHEDULE")
+        monkeypatch.delenv("GPUFEM_ILU_BATCH_N")
+    ilu.close()
+
+
 ILU_SCHEDULE_CASES = [
     # (name, mesh, ordering, level of fill, LDS budget or None, global rows)
     ("plate24_amd_ilu12", lambda: M.plate(24, 24, bc="ss", load="sine"),
