@@ -31,14 +31,16 @@ operator, which runs 1 + min(K, 3) steps).
 
 Extra fields: ``roofline`` (the dominant kernel, the sweeps, measured with
 HIP events on the library's stream; ``traffic`` is the HBM bytes per launch
-measured with rocprofv3 PMC counters for this workload and committed in
-profiles/, or null for other workloads), ``assembly_roofline`` (the FP64
-MFMA Jacobian kernel, with its MFMA rate), ``cpu_baseline`` (the oracle
+measured in this run by two rocprofv3 PMC passes over ``--profile-kernels``
+started as child processes (``pmc_traffic`` holds every hot kernel's
+figure), else the summary committed in profiles/ for this workload, else
+null), ``assembly_roofline`` (the FP64 MFMA Jacobian kernel, with its MFMA
+rate and PMC traffic), ``cpu_baseline`` (the oracle
 built -O3 -march=native on this host, per stage, one thread and all host
 threads, see ``sample``), ``configs_as_written`` (the plate with RCM +
 ILU(3) and the 256 x 512 cylinder with ILU(2), bounded), ``served`` (the
 served request mix at full size: plain FP64 one request at a time, then
-Poisson arrivals at 5x the plain rate through the server with worker
+Poisson arrivals at 10x the plain rate through the server with worker
 streams and with the batched GMRES engines) and ``latency_ms`` p50/p99 over
 the timed steps.  ``--mesh cylinder|wingbox`` runs the other workloads;
 ``--serve`` runs the served request mix alone (its own JSON line);
@@ -89,6 +91,8 @@ def parse():
     ap.add_argument("--restart", type=int, default=500)
     ap.add_argument("--max-iters", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the rocprofv3 PMC passes (HBM traffic)")
     # served request mix (separate JSON line; see serve_main)
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--requests", type=int, default=64)
@@ -97,7 +101,9 @@ def parse():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="Poisson arrival rate (requests/s); 0: "
                          "--rate-factor times the plain rate")
-    ap.add_argument("--rate-factor", type=float, default=5.0,
+    # arrivals faster than the server completes them (it runs saturated, so
+    # the served rate measures what it delivers, not the arrival rate)
+    ap.add_argument("--rate-factor", type=float, default=10.0,
                     help="arrival rate as a multiple of the measured plain "
                          "FP64 rate (0: all requests at t = 0)")
     ap.add_argument("--serve-engine", default="both",
@@ -109,7 +115,10 @@ def parse():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the configurations-as-written block")
     ap.add_argument("--serve-precision", type=int, default=2)
-    ap.add_argument("--serve-lev-fill", type=int, default=32)
+    # AMD + ILU(64): the cylinders of the mix take 34-64 iterations (ILU(32):
+    # 100-129); served 8.0 solves/s against 6.1 with ILU(32), plain FP64
+    # one at a time 1.46 against 1.70 solves/s
+    ap.add_argument("--serve-lev-fill", type=int, default=64)
     ap.add_argument("--serve-stall", type=int, default=4,
                     help="stall_cycles of the served requests")
     ap.add_argument("--seed", type=int, default=0)
@@ -164,6 +173,80 @@ def pmc_traffic(args, nodes, nnzb_ilu):
             vals.get("traffic_nnzb_ilu") != nnzb_ilu):
         return None
     return vals.get("traffic_bytes_per_launch")
+
+
+def live_pmc_traffic(args):
+    """HBM bytes of the hot kernels measured in this run: two rocprofv3 PMC
+    passes (FETCH_SIZE, then WRITE_SIZE; one counter a pass) over
+    ``--profile-kernels`` of the same workload, run as child processes, each
+    counter scaled by what it reads for the 1 GiB device-to-device copy that
+    run ends with.  Returns per-launch bytes of the sweeps (mean of L and U),
+    the SpMV, one assembly (its colour launches summed) and one numeric
+    factorisation (its launches summed), or None when rocprofv3 is missing or
+    a pass fails (the caller then falls back to the committed summary)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe or args.mesh != "plate":
+        return None
+    env = dict(os.environ, TMPDIR="/tmp")
+    per = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(td, ctr.lower())
+            cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", out,
+                   "-o", "run", "--", sys.executable, "-u",
+                   os.path.abspath(__file__), "--profile-kernels",
+                   "--nx", str(args.nx), "--reorder", args.reorder,
+                   "--lev-fill", str(args.lev_fill)]
+            print(f"[bench] live PMC pass {ctr}", file=sys.stderr, flush=True)
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=env, timeout=240,
+                                   capture_output=True, text=True)
+            except subprocess.TimeoutExpired:
+                return None
+            files = [os.path.join(dp, f) for dp, _, fs in os.walk(out)
+                     for f in fs if f.endswith("counter_collection.csv")]
+            if r.returncode != 0 or not files:
+                return None
+            for row in csv.DictReader(open(files[0])):
+                name = row["Kernel_Name"]
+                per.setdefault((name, ctr), []).append(
+                    float(row["Counter_Value"]))
+
+    def vals(pat, ctr):
+        return [v for (n, c), vs in per.items() if c == ctr and pat in n
+                for v in vs]
+
+    copy_f, copy_w = vals("copyBuffer", "FETCH_SIZE"), vals("copyBuffer",
+                                                          "WRITE_SIZE")
+    if not copy_f or not copy_w or max(copy_f) <= 0 or max(copy_w) <= 0:
+        return None
+    gib = float(1 << 30)
+    ff = gib / (max(copy_f) * 1024)  # the copy reads 1 GiB
+    wf = gib / (max(copy_w) * 1024)  # and writes 1 GiB
+
+    def bytes_of(pat, launches_per_unit=1):
+        f, w = vals(pat, "FETCH_SIZE"), vals(pat, "WRITE_SIZE")
+        if not f or not w:
+            return None
+        units = max(1, len(f) // launches_per_unit)
+        return (sum(f) * 1024 * ff + sum(w) * 1024 * wf) / units
+
+    lo = bytes_of("trsv_syncfree_kernel<double, true")
+    up = bytes_of("trsv_syncfree_kernel<double, false")
+    nasm = len(vals("jacobian_mfma_kernel<true>", "FETCH_SIZE"))
+    return {"sweep_bytes_per_launch": (None if lo is None or up is None
+                                       else (lo + up) / 2),
+            "sweep_lower": lo, "sweep_upper": up,
+            "spmv": bytes_of("spmv_kernel<double>"),
+            "assembly": bytes_of("jacobian_mfma_kernel<true>", max(1, nasm)),
+            "numeric_ilu": bytes_of("ilu_row_", 1 << 30),
+            "calibration": {"fetch_kib_per_gib": max(copy_f),
+                            "write_kib_per_gib": max(copy_w)}}
 
 
 def sweep_bytes(mrp, mcp, w):
@@ -364,6 +447,9 @@ def main():
     asm_bytes = 288 * nk + mesh.nelems * (16 + 64 + 56) + mesh.nnodes * (24 + 48 + 48)
     asm_gbs = asm_bytes / (asm_ms * 1e-3) / 1e9
 
+    # ---- HBM traffic of the hot kernels: rocprofv3 PMC passes (children)
+    pmc = None if args.no_pmc else live_pmc_traffic(args)
+
     # ---- CPU baseline: the oracle on the host, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
@@ -382,7 +468,7 @@ def main():
     if not args.no_configs and args.mesh == "plate" and nx == 400:
         as_written = configs_as_written(args, ctx)
 
-    # ---- served request mix (full size): Poisson arrivals at 5x the plain
+    # ---- served request mix (full size): Poisson arrivals at 10x the plain
     # FP64 rate, worker streams and the batched engines
     served = None
     if not args.no_serve:
@@ -428,7 +514,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "trsv_syncfree_kernel (L and U)",
                      "achieved": sweep_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sweep_gbs / HBM_PEAK_GBS,
-                     "traffic": pmc_traffic(args, nn, nm),
+                     "traffic": (pmc["sweep_bytes_per_launch"] if pmc
+                                 else pmc_traffic(args, nn, nm)),
+                     "traffic_source": (
+                         "rocprofv3 PMC passes in this run" if pmc else
+                         "profiles/pmc_sweep_plate400_amd_ilu"
+                         f"{args.lev_fill}.md" if pmc_traffic(args, nn, nm)
+                         else None),
                      "bytes_per_launch": sweep_bytes_launch,
                      "launch_us": sweep_launch_s * 1e6,
                      "levels": [info_lv["levels_lower"], info_lv["levels_upper"]]},
@@ -438,6 +530,7 @@ def main():
                               "achieved": asm_gbs, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": asm_gbs / HBM_PEAK_GBS,
                               "bytes_per_launch": asm_bytes,
+                              "traffic": pmc["assembly"] if pmc else None,
                               "launch_us": asm_ms * 1e3,
                               # FP64 MFMA rate of the element contraction
                               # (upper-triangle 4x4 tiles of K_e, k = 36)
@@ -448,6 +541,7 @@ def main():
                                   "peak_tflops": FP64_PEAK_TFLOPS,
                                   "frac": ASM_FLOPS_PER_ELEM * mesh.nelems
                                   / (asm_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}},
+        "pmc_traffic": pmc,
         "cpu_baseline": cpu,
         "configs_as_written": as_written,
         "served": served,
@@ -523,9 +617,10 @@ def serve_measure(args, nreq, scale, rate_factor, batch_modes):
     print(f"[serve] {len(reqs)} requests generated in {time.time()-t0:.1f}s; "
           f"nodes {min(m.nnodes for m in reqs)}..{max(m.nnodes for m in reqs)}",
           file=sys.stderr, flush=True)
-    # AMD + ILU(32) by default: AMD + ILU(3) stagnates near a relative
-    # residual of 1 on the thinner plates and cylinders of this mix, ILU(12)
-    # needs re-runs with more fill for the thinnest cylinders.
+    # AMD + ILU(64) by default (--serve-lev-fill): AMD + ILU(3) stagnates
+    # near a relative residual of 1 on the thinner plates and cylinders of
+    # this mix, ILU(12) needs re-runs with more fill for the thinnest
+    # cylinders, ILU(32) takes 100-129 iterations on the larger cylinders.
     # stall_cycles: a solve whose residual falls by less than 10% over
     # --serve-stall restart cycles stops early (and is re-run with twice the
     # level of fill when it stopped far from the tolerance)
