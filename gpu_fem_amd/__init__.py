@@ -41,6 +41,13 @@ class Options(ctypes.Structure):
                 ("stall_cycles", ctypes.c_int)]
 
 
+class NewtonInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int),
+                ("newton_iterations", ctypes.c_int),
+                ("krylov_iterations", ctypes.c_int),
+                ("rel_resid", ctypes.c_double), ("t_total", ctypes.c_double)]
+
+
 class SolveInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
                 ("outer", ctypes.c_int), ("rel_resid", ctypes.c_double),
@@ -79,6 +86,12 @@ _SIGS = {
     "gf_assemble_jacobian_colored_f64": [_P, _P, _P, _P, _P, _P, _P, _P, _P,
                                          _P],
     "gf_energy_mitc4_f64": [_P, _I, _P, _P, _P, _P, _P],
+    "gf_assemble_tangent_mitc4_f64": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P,
+                                      _P, _P],
+    "gf_assemble_tangent_colored_f64": [_P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                        _P],
+    "gf_vec_axpby_f64": [_P, ctypes.c_size_t, ctypes.c_double, _P,
+                         ctypes.c_double, _P],
     "gf_bc_apply_mat": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "gf_bc_apply_vec": [_P, _I, _P, _P, _P],
     "gf_bsr_copy_to_pattern": [_P, _I, _P, _P, _P, _P, _P, _P],
@@ -111,6 +124,9 @@ _SIGS = {
     "gf_problem_factor": [_P],
     "gf_problem_solve": [_P, _P, _P, _I, ctypes.POINTER(SolveInfo)],
     "gf_problem_sizes": [_P, _PI, _PI, _PI],
+    "gf_problem_assemble_tangent": [_P, _P, _P, _I],
+    "gf_problem_newton_solve": [_P, _P, _P, _I, _I, ctypes.c_double, _P,
+                                ctypes.POINTER(NewtonInfo)],
     "gf_problem_device_arrays": [_P] + [ctypes.POINTER(_P)] * 7,
     "gf_problem_host_perm": [_P, _P],
     "gf_problem_ilu": [_P],
@@ -515,6 +531,36 @@ class Problem:
         if code != GF_OK and raise_on_error:
             raise GfError(code, lib().gf_context_last_error(self.ctx.h).decode())
         return x, info
+
+    def assemble_tangent(self, u, res=None):
+        """Nonlinear tangent K(u) into the problem (with BCs) and, when res
+        is given (host or device, like u), the residual R(u)."""
+        on_dev = not isinstance(u, np.ndarray)
+        if not on_dev:
+            u = np.ascontiguousarray(u, np.float64)
+        self.ctx.check(lib().gf_problem_assemble_tangent(
+            self.h, ptr(u), ptr(res), int(on_dev)))
+        return res
+
+    def newton_solve(self, loads=None, u=None, max_newton=20,
+                     newton_rtol=1e-8, raise_on_error=True):
+        """Newton's method for the nonlinear shell, R(u) = loads; returns
+        (u, info, history of ||f - R(u)|| / ||f||)."""
+        loads = self.mesh.loads if loads is None else loads
+        on_dev = not isinstance(loads, np.ndarray)
+        if u is None:
+            u = (np.zeros(6 * self.mesh.nnodes) if not on_dev
+                 else loads.new_zeros(loads.shape))
+        if not on_dev:
+            loads = np.ascontiguousarray(loads, np.float64)
+        hist = np.zeros(max_newton + 1)
+        info = NewtonInfo()
+        code = lib().gf_problem_newton_solve(
+            self.h, ptr(loads), ptr(u), int(on_dev), max_newton, newton_rtol,
+            ptr(hist), ctypes.byref(info))
+        if code != GF_OK and raise_on_error:
+            raise GfError(code, lib().gf_context_last_error(self.ctx.h).decode())
+        return u, info, hist[:info.newton_iterations + 1]
 
     def close(self):
         if self.h:

@@ -727,6 +727,212 @@ __global__ void __launch_bounds__(64, GPUFEM_MFMA_MINWAVES)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Nonlinear MITC4 (IsotropicShell<..., isNonlinear = true>, physics.h:6-17):
+// NonlinearShellStrainCore (a2d/a2dshellstrain.h:72-93) adds to the bending
+// strains the quadratic terms of the local displacement gradients u0x, u1x
+// (basis.h:470-492),
+//   e3 += u0x(:,0).u1x(:,0),  e4 += u0x(:,1).u1x(:,1),
+//   e5 += u0x(:,0).u1x(:,1) + u1x(:,0).u0x(:,1);
+// the tying strains and the director stay linear (physics.h:214-219,
+// 286-300).  With G0, G1 the maps of the element DOFs to the entries
+// (r, c < 2) of u0x, u1x (mitc4::grad_columns_q), B_q(u) = dE_q/du and
+// S_q = detXd_q C E_q(u):
+//   R_e(u) = sum_q B_q(u)^T S_q                         (the gradient of
+//   U = sum_q 1/2 detXd_q E^T C E, physics.h:18-36),
+//   K_e(u) = sum_q detXd_q B_q(u)^T C B_q(u) + G0^T W_q G1 + G1^T W_q G0,
+// W_q holding S_q's bending stresses on the gradient-entry pairs each
+// quadratic term multiplies: K_e is the exact Hessian of U.  (The
+// reference's own nonlinear Jacobian routes its displacement gradients'
+// second-order terms into the residual, elem_group.h:286-289, and
+// overwrites the u0x sensitivity with the u1x one, basis.h:565-567.)
+// One element per 64-lane workgroup; per quadrature point lanes 0-23 form
+// column j of the linear B, G0 and G1 into LDS, 21 lanes form B u, G0 u and
+// G1 u, lane 0 the strain and stress, lanes 0-23 the column of B(u), C B(u)
+// and of W G1, W G0, and every lane adds a 3x3 tile of K_e.  Scatter as in
+// jacobian_kernel; R_e into res (6 perm[node] + k), the energy into
+// *energy; kvals, res or energy may be null.
+struct ElemNL {
+  double X[12];
+  double u[24];
+  double data[4];
+  mitc4::Geom g;
+  double B[9][24];   // B_q(u)
+  double D[9][24];   // detXd C B_q(u)
+  double G0[6][24];
+  double G1[6][24];
+  double P0[6][24];  // W G1
+  double P1[6][24];  // W G0
+  double eu[21];     // B_lin u, u0x entries, u1x entries
+  double S[9];       // detXd C E(u)
+};
+
+template <bool kColored>
+__global__ void __launch_bounds__(64)
+    tangent_kernel(int nelems, const int* __restrict__ elem_list,
+                   const int* __restrict__ first_mask,
+                   const int* __restrict__ conn,
+                   const double* __restrict__ xpts,
+                   const double* __restrict__ vars,
+                   const double* __restrict__ data,
+                   const int* __restrict__ elem_ind_map,
+                   const int* __restrict__ perm, double* kvals, double* res,
+                   double* energy) {
+  __shared__ ElemNL s;
+  if ((int)blockIdx.x >= nelems) return;
+  const int e = kColored ? elem_list[blockIdx.x] : (int)blockIdx.x;
+  const int mask = kColored ? first_mask[blockIdx.x] : 0;
+  const int lane = threadIdx.x;
+  if (lane < 12) {
+    s.X[lane] = xpts[3 * (size_t)conn[4 * (size_t)e + lane / 3] + lane % 3];
+  } else if (lane < 36) {
+    const int i = lane - 12;
+    s.u[i] = vars[6 * (size_t)conn[4 * (size_t)e + i / 6] + i % 6];
+  } else if (lane < 40) {
+    s.data[lane - 36] = data[4 * (size_t)e + (lane - 36)];
+  }
+  __syncthreads();
+  if (lane < 4)
+    mitc4::geom_node(lane, s.X, s.g);
+  else if (lane < 9)
+    mitc4::geom_tying_x(lane - 4, s.X, s.g);
+  __syncthreads();
+  if (lane < 4)
+    mitc4::geom_quad(lane, s.X, s.g);
+  else if (lane < 9)
+    mitc4::geom_tying_n(lane - 4, s.g);
+  __syncthreads();
+  mitc4::BColumnPrep prep;
+  if (lane < 24) mitc4::b_column_prep(lane, s.g, prep);
+  const int I = lane / 8, J = lane % 8;
+  double acc[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[r][c] = 0.0;
+  double racc = 0.0, uacc = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < 4; q++) {
+    const double sc = s.g.scale[q];
+    // linear B column and the gradient maps of DOF lane
+    double col[9], g0[6], g1[6];
+    if (lane < 24) {
+      mitc4::b_column_q(lane, q, s.g, prep, col);
+      mitc4::grad_columns_q(lane, q, s.g, g0, g1);
+#pragma unroll
+      for (int k = 0; k < 9; k++) s.B[k][lane] = col[k];
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        s.G0[m][lane] = g0[m];
+        s.G1[m][lane] = g1[m];
+      }
+    }
+    __syncthreads();
+    if (lane < 21) {
+      const double* row = lane < 9 ? s.B[lane]
+                          : lane < 15 ? s.G0[lane - 9] : s.G1[lane - 15];
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < 24; j++) v += row[j] * s.u[j];
+      s.eu[lane] = v;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      const double* U0 = &s.eu[9];
+      const double* U1 = &s.eu[15];
+      double E[9], st[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) E[k] = s.eu[k];
+      E[3] += U0[0] * U1[0] + U0[2] * U1[2] + U0[4] * U1[4];
+      E[4] += U0[1] * U1[1] + U0[3] * U1[3] + U0[5] * U1[5];
+      E[5] += U0[0] * U1[1] + U0[2] * U1[3] + U0[4] * U1[5] +
+              U1[0] * U0[1] + U1[2] * U0[3] + U1[4] * U0[5];
+      mitc4::stress(s.data, E, st);
+      double es = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        s.S[k] = sc * st[k];
+        es += E[k] * st[k];
+      }
+      uacc += 0.5 * sc * es;
+    }
+    __syncthreads();
+    if (lane < 24) {
+      const double* U0 = &s.eu[9];
+      const double* U1 = &s.eu[15];
+      // dE/du_lane: the linear column plus the quadratic terms' derivative
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        col[3] += g0[2 * r] * U1[2 * r] + U0[2 * r] * g1[2 * r];
+        col[4] += g0[2 * r + 1] * U1[2 * r + 1] + U0[2 * r + 1] * g1[2 * r + 1];
+        col[5] += g0[2 * r] * U1[2 * r + 1] + U0[2 * r] * g1[2 * r + 1] +
+                  g1[2 * r] * U0[2 * r + 1] + U1[2 * r] * g0[2 * r + 1];
+      }
+      double st[9];
+      mitc4::stress(s.data, col, st);
+      double rv = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        s.B[k][lane] = col[k];
+        s.D[k][lane] = sc * st[k];
+        rv += col[k] * s.S[k];
+      }
+      racc += rv;
+      const double S3 = s.S[3], S4 = s.S[4], S5 = s.S[5];
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        s.P0[2 * r][lane] = S3 * g1[2 * r] + S5 * g1[2 * r + 1];
+        s.P0[2 * r + 1][lane] = S5 * g1[2 * r] + S4 * g1[2 * r + 1];
+        s.P1[2 * r][lane] = S3 * g0[2 * r] + S5 * g0[2 * r + 1];
+        s.P1[2 * r + 1][lane] = S5 * g0[2 * r] + S4 * g0[2 * r + 1];
+      }
+    }
+    __syncthreads();
+    if (kvals) {
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const int i = 3 * I + r, j = 3 * J + c;
+          double v = 0.0;
+#pragma unroll
+          for (int k = 0; k < 9; k++) v += s.B[k][i] * s.D[k][j];
+#pragma unroll
+          for (int m = 0; m < 6; m++)
+            v += s.G0[m][i] * s.P0[m][j] + s.G1[m][i] * s.P1[m][j];
+          acc[r][c] += v;
+        }
+    }
+    __syncthreads();
+  }
+  if (kvals) {
+    const int a = I / 2, b = J / 2, r0 = 3 * (I % 2), c0 = 3 * (J % 2);
+    double* dst =
+        kvals + 36 * (size_t)elem_ind_map[16 * (size_t)e + 4 * a + b];
+    const bool first = (mask >> (4 * a + b)) & 1;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double* d = &dst[6 * (r0 + r) + c0 + c];
+        if (!kColored)
+          unsafeAtomicAdd(d, acc[r][c]);
+        else
+          *d = first ? acc[r][c] : *d + acc[r][c];
+      }
+  }
+  if (res && lane < 24) {
+    const int node = conn[4 * (size_t)e + lane / 6];
+    const int row = perm ? perm[node] : node;
+    double* d = &res[6 * (size_t)row + lane % 6];
+    if (!kColored)
+      unsafeAtomicAdd(d, racc);
+    else
+      *d = ((mask >> (16 + lane / 6)) & 1) ? racc : *d + racc;
+  }
+  if (energy && lane == 0) unsafeAtomicAdd(energy, uacc);
+}
+
 // Residual only: R_e = sum_q D_q^T (B_q u) (C symmetric).
 __global__ void __launch_bounds__(64)
     residual_kernel(int nelems, const int* __restrict__ conn,
@@ -960,6 +1166,49 @@ int gf_assemble_jacobian_colored_f64(gf_context ctx, gf_asm_plan pl,
                          perm, kvals, res);
   }
   return gf::check_launch(ctx, "jacobian_kernel<colored>");
+}
+
+int gf_assemble_tangent_mitc4_f64(gf_context ctx, int nelems, int nnodes,
+                                  const int* conn, const double* xpts,
+                                  const double* vars, const double* data,
+                                  const int* elem_ind_map, const int* perm,
+                                  double* kvals, double* res,
+                                  double* energy) {
+  if (!ctx || nelems < 0 || nnodes <= 0 || !conn || !xpts || !vars ||
+      !data || (kvals && !elem_ind_map))
+    return gf::set_error(ctx, GF_ERR_INVALID, "assemble_tangent: bad args");
+  if (energy)
+    GF_HIP(ctx, hipMemsetAsync(energy, 0, sizeof(double), ctx->stream));
+  if (nelems == 0) return GF_OK;
+  hipLaunchKernelGGL(gf::tangent_kernel<false>, dim3(nelems), dim3(64), 0,
+                     ctx->stream, nelems, (const int*)nullptr,
+                     (const int*)nullptr, conn, xpts, vars, data, elem_ind_map,
+                     perm, kvals, res, energy);
+  return gf::check_launch(ctx, "tangent_kernel");
+}
+
+int gf_assemble_tangent_colored_f64(gf_context ctx, gf_asm_plan pl,
+                                    const int* conn, const double* xpts,
+                                    const double* vars, const double* data,
+                                    const int* elem_ind_map, const int* perm,
+                                    double* kvals, double* res) {
+  if (!ctx || !pl || !conn || !xpts || !vars || !data || !elem_ind_map ||
+      !kvals)
+    return gf::set_error(ctx, GF_ERR_INVALID, "assemble_tangent_colored: bad args");
+  if (pl->zero_k)
+    GF_HIP(ctx, hipMemsetAsync(kvals, 0, sizeof(double) * 36 * (size_t)pl->nnzb,
+                               ctx->stream));
+  if (res && pl->zero_res)
+    GF_HIP(ctx, hipMemsetAsync(res, 0, sizeof(double) * 6 * (size_t)pl->nnodes,
+                               ctx->stream));
+  for (int c = 0; c < pl->ncolors; c++) {
+    const int c0 = pl->color_ptr[c], cnt = pl->color_ptr[c + 1] - c0;
+    hipLaunchKernelGGL(gf::tangent_kernel<true>, dim3(cnt), dim3(64), 0,
+                       ctx->stream, cnt, pl->elems + c0, pl->mask + c0, conn,
+                       xpts, vars, data, elem_ind_map, perm, kvals, res,
+                       (double*)nullptr);
+  }
+  return gf::check_launch(ctx, "tangent_kernel<colored>");
 }
 
 int gf_assemble_residual_mitc4_f64(gf_context ctx, int nelems, int nnodes,

@@ -245,6 +245,12 @@ __global__ void convert_kernel(size_t n, const double* __restrict__ in,
   if (i < n) out[i] = (float)in[i];
 }
 
+__global__ void axpby_kernel(size_t n, double a, const double* __restrict__ x,
+                             double b, double* __restrict__ y) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
 }  // namespace
 }  // namespace gf
 
@@ -427,6 +433,16 @@ int gf_bsr_resid_comp_f64(gf_context ctx, int nnodes, const int* rowp,
                      dim3(256), 0, ctx->stream, nnodes, rowp, cols, vals, x, b,
                      r);
   return gf::check_launch(ctx, "resid_comp_kernel");
+}
+
+int gf_vec_axpby_f64(gf_context ctx, size_t n, double a, const double* x,
+                     double b, double* y) {
+  if (!ctx || !x || !y)
+    return gf::set_error(ctx, GF_ERR_INVALID, "axpby: bad args");
+  if (n == 0) return GF_OK;
+  hipLaunchKernelGGL(gf::axpby_kernel, dim3(gf::div_up(n, 256)), dim3(256), 0,
+                     ctx->stream, n, a, x, b, y);
+  return gf::check_launch(ctx, "axpby_kernel");
 }
 
 int gf_convert_f64_f32(gf_context ctx, size_t n, const double* in,

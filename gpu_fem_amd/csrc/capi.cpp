@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -398,6 +399,125 @@ int gf_problem_solve(gf_problem p, const double* loads, double* x,
     info->t_factor = p->t_factor;
     info->t_solve = 1e-3 * elapsed(p->ev[0], p->ev[1]);
   }
+  return st;
+}
+
+int gf_problem_assemble_tangent(gf_problem p, const double* u, double* res,
+                                int is_device) {
+  if (!p || !u) return GF_ERR_INVALID;
+  gf_context ctx = p->ctx;
+  hipStream_t s = ctx->stream;
+  const size_t N = 6 * (size_t)p->nnodes;
+  GF_HIP(ctx, hipEventRecord(p->ev[0], s));
+  const double* du = u;
+  if (!is_device) {
+    GF_HIP(ctx, hipMemcpyAsync(p->work64, u, sizeof(double) * N,
+                               hipMemcpyHostToDevice, s));
+    du = p->work64;
+  }
+  // K(u) and R(u) (R at 6 perm[node] + k, in work64b)
+  GF_TRY(gf_assemble_tangent_colored_f64(ctx, p->plan, p->conn, p->xpts, du,
+                                         p->data, p->emap, p->perm, p->kvals,
+                                         p->work64b));
+  GF_TRY(gf_bc_apply_mat(ctx, p->nbcs, p->bcs, p->krowp, p->kcols, p->trowp,
+                         p->tcols, p->tmap, p->perm, p->kvals));
+  GF_TRY(gf_bsr_copy_to_pattern(ctx, p->nnodes, p->krowp, p->kcols, p->kvals,
+                                gf_ilu_device_rowp(p->ilu),
+                                gf_ilu_device_cols(p->ilu), p->mvals));
+  if (res) {
+    // unpermuted R: r[old] = R[perm[old]], constrained entries zero
+    GF_TRY(gf_permute_vec(ctx, p->nnodes, p->iperm, p->work64b, p->rhs));
+    GF_TRY(gf_bc_apply_vec(ctx, p->nbcs, p->bcs, nullptr, p->rhs));
+    GF_HIP(ctx, hipMemcpyAsync(res, p->rhs, sizeof(double) * N,
+                               is_device ? hipMemcpyDeviceToDevice
+                                         : hipMemcpyDeviceToHost,
+                               s));
+  }
+  GF_HIP(ctx, hipEventRecord(p->ev[1], s));
+  GF_HIP(ctx, hipEventSynchronize(p->ev[1]));
+  p->t_assemble = 1e-3 * elapsed(p->ev[0], p->ev[1]);
+  return GF_OK;
+}
+
+namespace {
+// ||x|| (device) through the problem's Krylov reduction (deterministic)
+int problem_norm(gf_problem p, const double* x, double* out) {
+  gf_context ctx = p->ctx;
+  GF_TRY(p->kw.dots(x, p->kw.N, 1, x, p->kw.scal, 2));
+  GF_HIP(ctx, hipMemcpyAsync(p->kw.host_scal, p->kw.scal, sizeof(double),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *out = p->kw.host_scal[0];
+  return GF_OK;
+}
+}  // namespace
+
+int gf_problem_newton_solve(gf_problem p, const double* loads, double* u,
+                            int is_device, int max_newton, double newton_rtol,
+                            double* history, gf_newton_info* info) {
+  if (!p || !loads || !u || max_newton < 0 || !(newton_rtol > 0.0))
+    return GF_ERR_INVALID;
+  gf_context ctx = p->ctx;
+  hipStream_t s = ctx->stream;
+  const size_t N = 6 * (size_t)p->nnodes;
+  const auto t0 = std::chrono::steady_clock::now();
+  double *f = nullptr, *x = nullptr, *r = nullptr, *dx = nullptr;
+  int st = gf::dmalloc(ctx, (void**)&f, sizeof(double) * N);
+  if (!st) st = gf::dmalloc(ctx, (void**)&x, sizeof(double) * N);
+  if (!st) st = gf::dmalloc(ctx, (void**)&r, sizeof(double) * N);
+  if (!st) st = gf::dmalloc(ctx, (void**)&dx, sizeof(double) * N);
+  gf_newton_info ni{};
+  ni.status = st;
+  auto run = [&]() -> int {
+    const hipMemcpyKind k =
+        is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    GF_HIP(ctx, hipMemcpyAsync(f, loads, sizeof(double) * N, k, s));
+    GF_HIP(ctx, hipMemcpyAsync(x, u, sizeof(double) * N, k, s));
+    GF_TRY(gf_bc_apply_vec(ctx, p->nbcs, p->bcs, nullptr, f));
+    GF_TRY(gf_bc_apply_vec(ctx, p->nbcs, p->bcs, nullptr, x));
+    double fn = 0.0;
+    GF_TRY(problem_norm(p, f, &fn));
+    if (fn == 0.0) fn = 1.0;
+    for (int it = 0;; it++) {
+      // r = f - R(x) with the tangent K(x) assembled and factored
+      GF_TRY(gf_problem_assemble_tangent(p, x, r, 1));
+      GF_TRY(gf_vec_axpby_f64(ctx, N, 1.0, f, -1.0, r));
+      double rn = 0.0;
+      GF_TRY(problem_norm(p, r, &rn));
+      ni.rel_resid = rn / fn;
+      if (history) history[it] = ni.rel_resid;
+      ni.newton_iterations = it;
+      if (!(ni.rel_resid > newton_rtol)) return GF_OK;
+      if (it == max_newton) return GF_ERR_NO_CONVERGENCE;
+      GF_TRY(gf_problem_factor(p));
+      gf_solve_info si{};
+      const int ss = gf_problem_solve(p, r, dx, 1, &si);
+      ni.krylov_iterations += si.iterations;
+      if (ss != GF_OK) return ss;
+      GF_TRY(gf_vec_axpby_f64(ctx, N, 1.0, dx, 1.0, x));
+    }
+  };
+  if (!st) {
+    st = run();
+    ni.status = st;
+    const int cs = hipMemcpyAsync(u, x, sizeof(double) * N,
+                                  is_device ? hipMemcpyDeviceToDevice
+                                            : hipMemcpyDeviceToHost,
+                                  s) == hipSuccess &&
+                           hipStreamSynchronize(s) == hipSuccess
+                       ? GF_OK
+                       : GF_ERR_DEVICE;
+    if (st == GF_OK && cs != GF_OK)
+      st = gf::set_error(ctx, cs, "newton_solve: copy-out failed");
+  }
+  gf::dfree(ctx, f);
+  gf::dfree(ctx, x);
+  gf::dfree(ctx, r);
+  gf::dfree(ctx, dx);
+  ni.t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                             t0)
+                   .count();
+  if (info) *info = ni;
   return st;
 }
 

@@ -392,6 +392,60 @@ __device__ __forceinline__ void b_column_q(int j, int q, const Geom& g,
   out[8] = et;
 }
 
+// Column j of the local displacement gradients at quadrature point q, the
+// entries (r, c), c = 0, 1, of u0x = T^T [U_xi | U_eta | d0] Xd^-1 T and
+// u1x = T^T ([d0_xi | d0_eta | 0] Xd^-1 T + [U_xi | U_eta | d0] Z)
+// (basis.h:470-492), in the order (0,0) (0,1) (1,0) (1,1) (2,0) (2,1):
+//   displacement DOF (a, k):  u0x_rc = t_r[k] (N_a,xi A_0c + N_a,eta A_1c)
+//                             u1x_rc = t_r[k] (N_a,xi Z_0c + N_a,eta Z_1c)
+//   rotation DOF (a, k):      u0x_rc = (fn_a x t_r)[k] N_a A_2c
+//                             u1x_rc = (fn_a x t_r)[k] (N_a,xi A_0c +
+//                                       N_a,eta A_1c + N_a Z_2c)
+// with A = Xd^-1 T, Z = -Xd^-1 Xdz Xd^-1 T and t_r the columns of T
+// (d_a = q_a x fn_a, director.h:98-107).  Entries (0,0), (1,1), (0,1) + (1,0)
+// of u1x are the linear bending strains of b_column_q.
+__device__ __forceinline__ void grad_columns_q(int j, int q, const Geom& g,
+                                               double g0[6], double g1[6]) {
+  const int a = j / 6, k = j % 6;
+  const bool rot = k >= 3;
+  const int kk = rot ? k - 3 : k;
+  const double xi = (q & 1) ? kRt3 : -kRt3;
+  const double eta = (q & 2) ? kRt3 : -kRt3;
+  double Na, Nxa, Nea;
+  shape_a(a, xi, eta, Na, Nxa, Nea);
+  const double* A = g.A[q];
+  const double* Z = g.Z[q];
+  const double t0[3] = {g.T[q][0], g.T[q][1], g.T[q][2]};
+  const double t1[3] = {g.T[q][3], g.T[q][4], g.T[q][5]};
+  double t2[3];
+  cross(t0, t1, t2);  // the normal column of the orthonormal T
+  double f[3];        // t_r[k] or (fn_a x t_r)[k]
+  if (!rot) {
+    f[0] = t0[kk];
+    f[1] = t1[kk];
+    f[2] = t2[kk];
+  } else {
+    double c[3];
+    cross(g.fn[a], t0, c);
+    f[0] = c[kk];
+    cross(g.fn[a], t1, c);
+    f[1] = c[kk];
+    cross(g.fn[a], t2, c);
+    f[2] = c[kk];
+  }
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const double pa = rot ? Na * A[6 + c] : Nxa * A[c] + Nea * A[3 + c];
+    const double pz = rot ? Nxa * A[c] + Nea * A[3 + c] + Na * Z[6 + c]
+                          : Nxa * Z[c] + Nea * Z[3 + c];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      g0[2 * r + c] = f[r] * pa;
+      g1[2 * r + c] = f[r] * pz;
+    }
+  }
+}
+
 // S = C E for one material point (a2d/a2disotropicshellstress.h:13-51).
 __device__ __forceinline__ void stress(const double d[4], const double e[9],
                                        double s[9]) {

@@ -126,6 +126,28 @@ int gf_assemble_residual_mitc4_f64(gf_context ctx, int nelems, int nnodes,
 int gf_energy_mitc4_f64(gf_context ctx, int nelems, const int* d_conn,
                         const double* d_xpts, const double* d_vars,
                         const double* d_data, double* d_energy);
+/* Nonlinear element, IsotropicShell<..., isNonlinear = true>
+ * (include/shell/physics.h:6-17, strain a2d/a2dshellstrain.h:72-93): the
+ * tangent K(u) (the Hessian of the strain energy at d_vars) added into
+ * d_kvals through elemIndMap, the residual R(u) (its gradient) added at
+ * 6*perm[node]+k into d_res, the energy (physics.h:18-36) written to
+ * *d_energy; replaces add_jacobian / add_residual / add_energy of the
+ * nonlinear element group (include/base/elem_group.cuh:101-211,
+ * include/base/elem_group.h:40-181).  d_kvals, d_res and d_energy may each
+ * be NULL; d_kvals and d_res are not zeroed (FP64 atomic adds). */
+int gf_assemble_tangent_mitc4_f64(gf_context ctx, int nelems, int nnodes,
+                                  const int* d_conn, const double* d_xpts,
+                                  const double* d_vars, const double* d_data,
+                                  const int* d_elem_ind_map,
+                                  const int* d_perm, double* d_kvals,
+                                  double* d_res, double* d_energy);
+/* the same on a colouring plan (no atomics, no zeroing; deterministic) */
+int gf_assemble_tangent_colored_f64(gf_context ctx, gf_asm_plan plan,
+                                    const int* d_conn, const double* d_xpts,
+                                    const double* d_vars, const double* d_data,
+                                    const int* d_elem_ind_map,
+                                    const int* d_perm, double* d_kvals,
+                                    double* d_res);
 /* zero rows and columns of constrained DOFs, unit diagonal
  * (bsr_mat.cuh:2-104 apply_mat_bcs_rows/cols_kernel) */
 int gf_bc_apply_mat(gf_context ctx, int nbcs, const int* d_bcs,
@@ -154,6 +176,9 @@ int gf_bsr_spmv_f32(gf_context ctx, int nnodes, const int* d_rowp,
                     float* d_y);
 int gf_convert_f64_f32(gf_context ctx, size_t n, const double* d_in,
                        float* d_out);
+/* y = a x + b y (cublasDaxpy / cublasDscal, gmres.cu:581-666) */
+int gf_vec_axpby_f64(gf_context ctx, size_t n, double a, const double* d_x,
+                     double b, double* d_y);
 /* r = b - A x accumulated with error-free transformations (compensated
  * dot products), as accurate as twice-FP64 arithmetic rounded once; the true
  * residual of the GMRES outer loop.  r may alias b but not x. */
@@ -261,6 +286,31 @@ int gf_problem_factor(gf_problem p);
  * 6*nnodes doubles (device when is_device != 0) */
 int gf_problem_solve(gf_problem p, const double* loads, double* x,
                      int is_device, gf_solve_info* info);
+/* Nonlinear (Newton) path, IsotropicShell<..., isNonlinear = true>
+ * (include/shell/physics.h:6-17).  gf_problem_assemble_tangent: K(u) with
+ * the boundary conditions into the problem's K and preconditioner values
+ * (the nonlinear add_jacobian + apply_bcs, assembler.h:267-305), and the
+ * residual R(u) into res (unpermuted, constrained entries zero; may be
+ * NULL); u and res are host or device arrays of 6*nnodes doubles. */
+int gf_problem_assemble_tangent(gf_problem p, const double* u, double* res,
+                                int is_device);
+typedef struct {
+  int status;             /* GF_OK or an error code */
+  int newton_iterations;  /* tangent solves taken */
+  int krylov_iterations;  /* GMRES iterations over all of them */
+  double rel_resid;       /* ||f - R(u)|| / ||f||, constrained entries zero */
+  double t_total;         /* seconds (host clock) */
+} gf_newton_info;
+/* Newton's method for R(u) = f: from the initial u (in/out), each step
+ * assembles and factors the tangent K(u) and solves K du = f - R(u) with the
+ * problem's preconditioned GMRES (its rtol, fill and precision), until
+ * ||f - R(u)|| <= newton_rtol ||f|| or max_newton steps (then
+ * GF_ERR_NO_CONVERGENCE).  history (may be NULL) receives the relative
+ * residual before each step, max_newton + 1 doubles.  The reference has no
+ * Newton driver; this is the loop its nonlinear element group is for. */
+int gf_problem_newton_solve(gf_problem p, const double* loads, double* u,
+                            int is_device, int max_newton, double newton_rtol,
+                            double* history, gf_newton_info* info);
 /* sizes and device pointers for tests and benchmarks */
 int gf_problem_sizes(gf_problem p, int* nnodes, int* nnzb_k, int* nnzb_m);
 int gf_problem_device_arrays(gf_problem p, const int** krowp,

@@ -83,6 +83,16 @@ def _load(path):
                                         ctypes.POINTER(ctypes.c_double)]
         L.orc_mitc4_strain.argtypes = [ctypes.c_int, _f64p, _f64p, _f64p]
         L.orc_mitc4_strain.restype = ctypes.c_double
+        L.orc_mitc4_strain_nl.argtypes = [ctypes.c_int, _f64p, _f64p, _f64p,
+                                          _f64p, _f64p]
+        L.orc_mitc4_strain_nl.restype = ctypes.c_double
+        L.orc_mitc4_element_nl.argtypes = [_f64p, _f64p, _f64p,
+                                           ctypes.c_void_p, _f64p,
+                                           ctypes.POINTER(ctypes.c_double)]
+        L.orc_assemble_tangent.argtypes = [
+            ctypes.c_int, _i32p, _f64p, _f64p, _f64p, _i32p,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.POINTER(ctypes.c_double)]
         L.orc_node_graph.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      _i32p, _pp, _pp]
         L.orc_rcm.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.c_int, _i32p]
@@ -185,6 +195,44 @@ def strain(iq, xpts, vars):
     E = np.zeros(9)
     s = lib().orc_mitc4_strain(iq, _f64(xpts).ravel(), _f64(vars).ravel(), E)
     return E, s
+
+
+def strain_nl(iq, xpts, vars):
+    """Nonlinear strain E (9), local displacement gradients u0x, u1x (3 x 3)
+    and detXd * weight at quadrature point iq."""
+    E, u0x, u1x = np.zeros(9), np.zeros(9), np.zeros(9)
+    s = lib().orc_mitc4_strain_nl(iq, _f64(xpts).ravel(), _f64(vars).ravel(),
+                                  E, u0x, u1x)
+    return E, u0x.reshape(3, 3), u1x.reshape(3, 3), s
+
+
+def element_nl(xpts, vars, data=(70e9, 0.3, 0.005, 0.0)):
+    """Nonlinear element at vars: tangent K (24x24), residual R (24), energy."""
+    K = np.zeros(576)
+    R = np.zeros(24)
+    U = ctypes.c_double()
+    lib().orc_mitc4_element_nl(_f64(xpts).ravel(), _f64(vars).ravel(),
+                               _f64(data), K.ctypes.data_as(ctypes.c_void_p),
+                               R, ctypes.byref(U))
+    return K.reshape(24, 24), R, U.value
+
+
+def assemble_tangent(conn, xpts, data, elem_ind_map, perm, nnzb, vars,
+                     with_k=True):
+    """Nonlinear K(u) (BSR values through elem_ind_map), R(u) at
+    6 perm[node] + k, and the energy."""
+    conn = _i32(conn).ravel()
+    ne = len(conn) // 4
+    nnodes = len(np.asarray(xpts).ravel()) // 3
+    kvals = np.zeros(36 * nnzb) if with_k else None
+    res = np.zeros(6 * nnodes)
+    U = ctypes.c_double()
+    perm = None if perm is None else _i32(perm)
+    lib().orc_assemble_tangent(
+        ne, conn, _f64(xpts).ravel(), _f64(vars).ravel(), _f64(data).ravel(),
+        _i32(elem_ind_map), _ptr(perm), _ptr(kvals), _ptr(res),
+        ctypes.byref(U))
+    return kvals, res, U.value
 
 
 # ---------------------------------------------------------------- symbolic

@@ -254,11 +254,13 @@ void sym_rotate(const double A[9], const double g[6], double e[6]) {
 
 }  // namespace
 
-// Strain vector E (9) at quadrature point iq; returns detXd * weight.
-// Follows add_element_quadpt_residual's forward block
+namespace orc_internal {
+// Strain vector E (9) at quadrature point iq and the local displacement
+// gradients u0x, u1x (3 x 3 row-major, basis.h:470-492); returns
+// detXd * weight.  Follows add_element_quadpt_residual's forward block
 // (include/shell/elem_group.h:123-160).
-extern "C" double orc_mitc4_strain(int iq, const double X[12],
-                                   const double v[24], double E[9]) {
+double strain_full(int iq, const double X[12], const double v[24],
+                   double E[9], double u0xl[9], double u1xl[9]) {
   double pt[2] = {iq % 2 == 0 ? -kRt3 : kRt3, iq / 2 == 0 ? -kRt3 : kRt3};
   double weight = 1.0 * 1.0;
   double fn[12];
@@ -296,6 +298,10 @@ extern "C" double orc_mitc4_strain(int iq, const double X[12],
   mm(u0x, XdinvzT, tmp2);
   for (int i = 0; i < 9; i++) tmp[i] += tmp2[i];
   mm(T, tmp, u1x, true, false);
+  // u0x = T^T u0d Xd^-1 T (basis.h:490-492)
+  mm(u0x, XdinvT, tmp);
+  mm(T, tmp, u0xl, true, false);
+  for (int i = 0; i < 9; i++) u1xl[i] = u1x[i];
   double g[6], e0ty[6];
   interp_tying(pt, ety, g);
   sym_rotate(XdinvT, g, e0ty);
@@ -311,6 +317,33 @@ extern "C" double orc_mitc4_strain(int iq, const double X[12],
   E[7] = 2.0 * e0ty[2];
   E[8] = et;
   return detXd * weight;
+}
+
+// NonlinearShellStrainCore (a2d/a2dshellstrain.h:72-93): the bending
+// strains gain the quadratic terms of the displacement gradients (row-major
+// u0x, u1x): k11 += u0x(:,0).u1x(:,0), k22 += u0x(:,1).u1x(:,1),
+// k12 += u0x(:,0).u1x(:,1) + u1x(:,0).u0x(:,1).  The tying strains and the
+// director stay linear (physics.h:214-219, 286-300).
+void add_nonlinear(const double u0x[9], const double u1x[9], double E[9]) {
+  E[3] += u0x[0] * u1x[0] + u0x[3] * u1x[3] + u0x[6] * u1x[6];
+  E[4] += u0x[1] * u1x[1] + u0x[4] * u1x[4] + u0x[7] * u1x[7];
+  E[5] += u0x[0] * u1x[1] + u0x[3] * u1x[4] + u0x[6] * u1x[7] +
+          u1x[0] * u0x[1] + u1x[3] * u0x[4] + u1x[6] * u0x[7];
+}
+}  // namespace orc_internal
+
+extern "C" double orc_mitc4_strain(int iq, const double X[12],
+                                   const double v[24], double E[9]) {
+  double u0x[9], u1x[9];
+  return orc_internal::strain_full(iq, X, v, E, u0x, u1x);
+}
+
+extern "C" double orc_mitc4_strain_nl(int iq, const double X[12],
+                                      const double v[24], double E[9],
+                                      double u0x[9], double u1x[9]) {
+  const double sc = orc_internal::strain_full(iq, X, v, E, u0x, u1x);
+  orc_internal::add_nonlinear(u0x, u1x, E);
+  return sc;
 }
 
 namespace {
@@ -383,6 +416,81 @@ extern "C" void orc_mitc4_element(const double X[12], const double vars[24],
         R[j] += scale[q] * acc;
       }
     }
+  }
+  if (energy) *energy = U;
+}
+
+// Nonlinear element (IsotropicShell<..., isNonlinear = true>, physics.h:6-17):
+// the energy U = sum_q 1/2 detXd_q E_q(u)^T C E_q(u) with E_q from
+// NonlinearShellStrainCore, its gradient R and its Hessian K, formed from the
+// linear maps (unit DOF vectors through the forward maps) of the linear
+// strain, u0x and u1x:
+//   dE_s/du_j = L_sj + [s = 3..5] d(quadratic)/du_j,
+//   d2E_s/du_i du_j = sum over the pairs (m, m') the quadratic term of s
+//                     multiplies of u0x_m(e_i) u1x_m'(e_j) + u0x_m(e_j) u1x_m'(e_i),
+//   R_j = sum_q detXd_q (dE/du_j) . C E,
+//   K_ij = sum_q detXd_q [(dE/du_i) . C (dE/du_j) + sum_s (C E)_s d2E_s/du_i du_j].
+extern "C" void orc_mitc4_element_nl(const double X[12], const double vars[24],
+                                     const double data[4], double* K,
+                                     double* R, double* energy) {
+  if (K) std::memset(K, 0, sizeof(double) * 576);
+  if (R) std::memset(R, 0, sizeof(double) * 24);
+  double U = 0.0;
+  // (m, m') pairs of the quadratic terms of E3, E4, E5 (row-major indices)
+  const int p3[3][2] = {{0, 0}, {3, 3}, {6, 6}};
+  const int p4[3][2] = {{1, 1}, {4, 4}, {7, 7}};
+  const int p5[6][2] = {{0, 1}, {3, 4}, {6, 7}, {1, 0}, {4, 3}, {7, 6}};
+  for (int q = 0; q < 4; q++) {
+    double Eu[9], U0[9], U1[9];
+    const double sc = orc_mitc4_strain_nl(q, X, vars, Eu, U0, U1);
+    double L[24][9], G0[24][9], G1[24][9], unit[24];
+    for (int j = 0; j < 24; j++) {
+      std::memset(unit, 0, sizeof(unit));
+      unit[j] = 1.0;
+      orc_internal::strain_full(q, X, unit, L[j], G0[j], G1[j]);
+    }
+    // dE/du_j
+    double Bt[24][9];
+    for (int j = 0; j < 24; j++) {
+      for (int s = 0; s < 9; s++) Bt[j][s] = L[j][s];
+      for (int t = 0; t < 3; t++) {
+        Bt[j][3] += G0[j][p3[t][0]] * U1[p3[t][1]] + U0[p3[t][0]] * G1[j][p3[t][1]];
+        Bt[j][4] += G0[j][p4[t][0]] * U1[p4[t][1]] + U0[p4[t][0]] * G1[j][p4[t][1]];
+      }
+      for (int t = 0; t < 6; t++)
+        Bt[j][5] += G0[j][p5[t][0]] * U1[p5[t][1]] + U0[p5[t][0]] * G1[j][p5[t][1]];
+    }
+    double Su[9];
+    stress(data, Eu, Su);
+    double eS = 0.0;
+    for (int s = 0; s < 9; s++) eS += Eu[s] * Su[s];
+    U += 0.5 * sc * eS;
+    if (R)
+      for (int j = 0; j < 24; j++) {
+        double acc = 0.0;
+        for (int s = 0; s < 9; s++) acc += Bt[j][s] * Su[s];
+        R[j] += sc * acc;
+      }
+    if (K)
+      for (int j = 0; j < 24; j++) {
+        double Sj[9];
+        stress(data, Bt[j], Sj);
+        for (int i = 0; i < 24; i++) {
+          double acc = 0.0;
+          for (int s = 0; s < 9; s++) acc += Bt[i][s] * Sj[s];
+          double geo = 0.0;
+          for (int t = 0; t < 3; t++) {
+            geo += Su[3] * (G0[i][p3[t][0]] * G1[j][p3[t][1]] +
+                            G0[j][p3[t][0]] * G1[i][p3[t][1]]);
+            geo += Su[4] * (G0[i][p4[t][0]] * G1[j][p4[t][1]] +
+                            G0[j][p4[t][0]] * G1[i][p4[t][1]]);
+          }
+          for (int t = 0; t < 6; t++)
+            geo += Su[5] * (G0[i][p5[t][0]] * G1[j][p5[t][1]] +
+                            G0[j][p5[t][0]] * G1[i][p5[t][1]]);
+          K[24 * i + j] += sc * (acc + geo);
+        }
+      }
   }
   if (energy) *energy = U;
 }

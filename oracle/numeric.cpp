@@ -7,6 +7,8 @@
 //                          elemIndMap; residual into 6*perm[conn]+k as the
 //                          device path vec.h:278-293 does
 //   orc_assemble_residual  include/base/elem_group.h:40-79 add_residual_cpu
+//   orc_assemble_tangent   the same loop for the nonlinear element
+//                          (IsotropicShell<..., isNonlinear = true>)
 //   orc_apply_bcs_mat      bsr_mat.cuh:2-104 (row and column kernels)
 //   orc_apply_bcs_vec      vec.cuh:4-13 (optionally through perm)
 //   orc_copy_values        bsr_mat.cuh:107-140 copy_mat_values_kernel
@@ -107,6 +109,43 @@ extern "C" void orc_assemble_jacobian(int nelems, const int* conn,
         for (int k = 0; k < 6; k++) res[6 * nd + k] += Re[6 * a + k];
       }
   }
+}
+
+// Nonlinear tangent K(u) and residual R(u) (orc_mitc4_element_nl), gathered
+// and scattered as orc_assemble_jacobian; energy summed when requested.
+extern "C" void orc_assemble_tangent(int nelems, const int* conn,
+                                     const double* xpts, const double* vars,
+                                     const double* data,
+                                     const int* elem_ind_map, const int* perm,
+                                     double* kvals, double* res,
+                                     double* energy) {
+  std::vector<double> Ke(576), Re(24);
+  double U = 0.0;
+  for (int e = 0; e < nelems; e++) {
+    double xe[12], ve[24], Ue = 0.0;
+    for (int a = 0; a < 4; a++) {
+      int nd = conn[4 * e + a];
+      for (int k = 0; k < 3; k++) xe[3 * a + k] = xpts[3 * nd + k];
+      for (int k = 0; k < 6; k++) ve[6 * a + k] = vars[6 * nd + k];
+    }
+    orc_mitc4_element_nl(xe, ve, &data[4 * e], kvals ? Ke.data() : nullptr,
+                         Re.data(), &Ue);
+    U += Ue;
+    if (kvals)
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+          double* blk = &kvals[36 * (size_t)elem_ind_map[16 * e + 4 * a + b]];
+          for (int r = 0; r < 6; r++)
+            for (int c = 0; c < 6; c++)
+              blk[6 * r + c] += Ke[24 * (6 * a + r) + 6 * b + c];
+        }
+    if (res)
+      for (int a = 0; a < 4; a++) {
+        int nd = perm ? perm[conn[4 * e + a]] : conn[4 * e + a];
+        for (int k = 0; k < 6; k++) res[6 * nd + k] += Re[6 * a + k];
+      }
+  }
+  if (energy) *energy = U;
 }
 
 extern "C" void orc_assemble_residual(int nelems, const int* conn,

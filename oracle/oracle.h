@@ -32,6 +32,19 @@ void orc_mitc4_element(const double xpts[12], const double vars[24],
 /* 9 strains and detXd*weight at quadrature point iq for given element vars. */
 double orc_mitc4_strain(int iq, const double xpts[12], const double vars[24],
                         double E[9]);
+// nonlinear strain (a2d/a2dshellstrain.h:72-93) and the local displacement
+// gradients u0x, u1x (row-major 3 x 3) at quadrature point iq
+double orc_mitc4_strain_nl(int iq, const double xpts[12],
+                           const double vars[24], double E[9], double u0x[9],
+                           double u1x[9]);
+// nonlinear element: K = Hessian, R = gradient of the energy at vars
+void orc_mitc4_element_nl(const double xpts[12], const double vars[24],
+                          const double data[4], double* K, double* R,
+                          double* energy);
+void orc_assemble_tangent(int nelems, const int* conn, const double* xpts,
+                          const double* vars, const double* data,
+                          const int* elem_ind_map, const int* perm,
+                          double* kvals, double* res, double* energy);
 
 /* ---- symbolic (include/linalg/bsr_utils.h, reordering_utils.h) ---- */
 int orc_node_graph(int nelems, int nnodes, int nodes_per_elem,
