@@ -584,11 +584,14 @@ struct ShellIsotropicData {
       : E(E_), nu(nu_), thick(thick_), tOffset(tOffset_) {}
   T E = 0, nu = 0, thick = 0, tOffset = 0;
 };
+// include/shell/physics.h:6-17: isNonlinear selects the strain of
+// NonlinearShellStrainCore (a2d/a2dshellstrain.h:72-93); the assembler then
+// forms the exact tangent and residual at the current variables
 template <typename T, class Data_, bool isNonlinear = false>
 struct IsotropicShell {
-  static_assert(!isNonlinear, "linear strains only");
   using Data = Data_;
   static constexpr int vars_per_node = 6;
+  static constexpr bool is_nonlinear = isNonlinear;
 };
 template <typename T, class Director_, class Basis_, class Phys_>
 struct ShellElementGroup {
@@ -755,33 +758,55 @@ class ElementAssembler {
 
   void add_energy(T& Uenergy, bool = false) {
     DeviceVec<T> e(1);
-    check(gf_energy_mitc4_f64(context(), num_elements, conn_.getPtr(),
-                              xpts_.getPtr(), vars_.getPtr(), data_.getPtr(),
-                              e.getPtr()),
-          "add_energy");
+    if constexpr (Phys::is_nonlinear)
+      check(gf_assemble_tangent_mitc4_f64(
+                context(), num_elements, num_vars_nodes, conn_.getPtr(),
+                xpts_.getPtr(), vars_.getPtr(), data_.getPtr(), nullptr,
+                nullptr, nullptr, nullptr, e.getPtr()),
+            "add_energy");
+    else
+      check(gf_energy_mitc4_f64(context(), num_elements, conn_.getPtr(),
+                                xpts_.getPtr(), vars_.getPtr(), data_.getPtr(),
+                                e.getPtr()),
+            "add_energy");
     Uenergy = e.createHostVec()[0];
   }
   // residual accumulated at 6*perm[node]+k of the matrix numbering
   void add_residual(Vec<T>& res, bool = false) {
     const int* perm = device_perm();
-    check(gf_assemble_residual_mitc4_f64(context(), num_elements,
-                                         num_vars_nodes, conn_.getPtr(),
-                                         xpts_.getPtr(), vars_.getPtr(),
-                                         data_.getPtr(), perm, res.getPtr()),
-          "add_residual");
+    if constexpr (Phys::is_nonlinear)
+      check(gf_assemble_tangent_mitc4_f64(
+                context(), num_elements, num_vars_nodes, conn_.getPtr(),
+                xpts_.getPtr(), vars_.getPtr(), data_.getPtr(), nullptr, perm,
+                nullptr, res.getPtr(), nullptr),
+            "add_residual");
+    else
+      check(gf_assemble_residual_mitc4_f64(context(), num_elements,
+                                           num_vars_nodes, conn_.getPtr(),
+                                           xpts_.getPtr(), vars_.getPtr(),
+                                           data_.getPtr(), perm, res.getPtr()),
+            "add_residual");
   }
-  // K accumulated into mat (zeroed here), residual into res
+  // K accumulated into mat (zeroed here), residual into res; the tangent
+  // K(u) and R(u) at the current variables for the nonlinear element
   void add_jacobian(Vec<T>& res, Mat& mat, bool = false) {
     BsrData d = mat.getBsrData();
     if (d.host || !d.elemIndMap)
       throw Error(GF_ERR_INVALID,
                   "add_jacobian: matrix needs device BsrData with elemIndMap");
     mat.zeroValues();
-    check(gf_assemble_jacobian_mitc4_f64(
-              context(), num_elements, num_vars_nodes, conn_.getPtr(),
-              xpts_.getPtr(), vars_.getPtr(), data_.getPtr(), d.elemIndMap,
-              d.perm, mat.getPtr(), res.getPtr()),
-          "add_jacobian");
+    if constexpr (Phys::is_nonlinear)
+      check(gf_assemble_tangent_mitc4_f64(
+                context(), num_elements, num_vars_nodes, conn_.getPtr(),
+                xpts_.getPtr(), vars_.getPtr(), data_.getPtr(), d.elemIndMap,
+                d.perm, mat.getPtr(), res.getPtr(), nullptr),
+            "add_jacobian");
+    else
+      check(gf_assemble_jacobian_mitc4_f64(
+                context(), num_elements, num_vars_nodes, conn_.getPtr(),
+                xpts_.getPtr(), vars_.getPtr(), data_.getPtr(), d.elemIndMap,
+                d.perm, mat.getPtr(), res.getPtr()),
+            "add_jacobian");
     perm_cache_ = d;
   }
 
