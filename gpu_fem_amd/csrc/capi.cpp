@@ -151,7 +151,7 @@ int problem_setup(gf_problem p, gf::ProblemPlan& pl, const int* conn,
     GF_TRY(dalloc(ctx, &p->t32b, N));
     GF_TRY(dalloc(ctx, &p->t32c, N));
   }
-  GF_TRY(p->kw.init(ctx, N, p->opt.restart, mixed));
+  GF_TRY(p->kw.init(ctx, N, p->opt.restart, mixed, p->opt.precision != 1));
   GF_HIP(ctx, hipEventCreate(&p->ev[0]));
   GF_HIP(ctx, hipEventCreate(&p->ev[1]));
   GF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -236,8 +236,10 @@ size_t ProblemPlan::device_bytes(bool caching) const {
     add(36 * nm * F);
     for (int i = 0; i < 3; i++) add(N * F);
   }
-  // Krylov work space (KrylovWork::init)
+  // Krylov work space (KrylovWork::init): basis and, outside GMRES-IR, its
+  // preconditioned images
   add(N * (m + 1) * D);
+  if (opt.precision != 1) add(N * (m > 0 ? m : 1) * D);
   for (int i = 0; i < 4; i++) add(N * D);
   add((m + 2) * D); add((m + 2) * D); add(2 * D); add((m + 1) * D);
   add(((m + 1) * m + 3 * m + 2) * D); add(256 * (2 * m + 4) * D);
@@ -577,7 +579,7 @@ extern "C" int gf_gmres_solve(gf_context ctx, int nnodes, const int* krowp,
   gf::KrylovWork kw;
   double *w64 = nullptr, *w64b = nullptr;
   float *a = nullptr, *bb = nullptr, *c = nullptr;
-  int st = kw.init(ctx, N, restart, mixed);
+  int st = kw.init(ctx, N, restart, mixed, precision != 1);
   if (!st) st = gf::dmalloc(ctx, (void**)&w64, sizeof(double) * N);
   if (!st) st = gf::dmalloc(ctx, (void**)&w64b, sizeof(double) * N);
   if (!st && mixed) {

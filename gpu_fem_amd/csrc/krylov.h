@@ -22,8 +22,12 @@ struct KrylovWork {
   double *hc = nullptr, *dd = nullptr, *co = nullptr, *rho = nullptr;
   float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
   double* host_scal = nullptr;  // pinned: two (estimate, h_{j+1,j}) slots
+  // Z_i = M^-1 V_i kept beside the basis (m vectors; null when not kept):
+  // the cycle's update is then x += Z y, without a preconditioner
+  // application
+  double* Z = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
-  int init(gf_context c, size_t N, int m, bool mixed);
+  int init(gf_context c, size_t N, int m, bool mixed, bool keep_z = false);
   void release();
   // out[0..nvec) = V_i . x (mode 0 store, 1 add, 2 sqrt); skipped when
   // *gate == 0 (a stored result then reads zero)
@@ -48,7 +52,10 @@ struct Operator {
   // prec: 0 FP64 ILU; 1 FP32 ILU and FP32 SpMV (inner loop of GMRES-IR);
   // 2 FP32 ILU, FP64 SpMV
   int apply_prec(const double* in, double* out, int prec) const;
-  int apply_prec_a(const double* in, double* out, int prec) const;
+  // out = A M^-1 in; z_out (may be null, precisions 0 and 2) receives
+  // M^-1 in in FP64
+  int apply_prec_a(const double* in, double* out, int prec,
+                   double* z_out = nullptr) const;
 };
 
 struct GmresResult {

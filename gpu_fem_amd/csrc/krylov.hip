@@ -455,6 +455,32 @@ __device__ __forceinline__ void update2_body(size_t t, int j, int m,
   V[(size_t)(j + 1) * ldv + t] = (w[t] - ac - cc[j] * qf) * co[2 * m + 3];
 }
 
+// update2 with the same correction applied to Z_j = M^-1 V_j (the
+// preconditioned vectors follow the basis they were computed from)
+// zj = M^-1 V_j as computed from the uncorrected V_j
+__device__ __forceinline__ void update_z_body(size_t t, int j, int m,
+                                              double zj,
+                                              double* __restrict__ Z,
+                                              size_t ldv,
+                                              const double* __restrict__ co) {
+  if (j > 0) {
+    double az = 0.0;
+    for (int i = 0; i < j; i++) az += co[i] * Z[(size_t)i * ldv + t];
+    zj = (zj - az) * co[2 * m + 2];
+  }
+  Z[(size_t)j * ldv + t] = zj;
+}
+
+__global__ void update2z(size_t N, int j, int m, double* __restrict__ V,
+                         size_t ldv, const double* __restrict__ w,
+                         const double* __restrict__ co,
+                         double* __restrict__ Z) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  update_z_body(t, j, m, Z[(size_t)j * ldv + t], Z, ldv, co);
+  update2_body(t, j, m, V, ldv, w, co);
+}
+
 __global__ void update2(size_t N, int j, int m, double* __restrict__ V,
                         size_t ldv, const double* __restrict__ w,
                         const double* __restrict__ co) {
@@ -467,7 +493,8 @@ inline dim3 grid1(size_t N) { return dim3(div_up(N, 256)); }
 
 }  // namespace
 
-int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
+int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed,
+                     bool keep_z) {
   ctx = c;
   N = N_;
   m = m_;
@@ -475,6 +502,7 @@ int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
     return dmalloc(ctx, p, bytes ? bytes : 8);
   };
   GF_TRY(dm((void**)&V, sizeof(double) * N * (m + 1)));
+  if (keep_z) GF_TRY(dm((void**)&Z, sizeof(double) * N * (m > 0 ? m : 1)));
   GF_TRY(dm((void**)&w, sizeof(double) * N));
   GF_TRY(dm((void**)&z, sizeof(double) * N));
   GF_TRY(dm((void**)&r, sizeof(double) * N));
@@ -509,6 +537,7 @@ int KrylovWork::init(gf_context c, size_t N_, int m_, bool mixed) {
 
 void KrylovWork::release() {
   dfree(ctx, V);
+  dfree(ctx, Z);
   dfree(ctx, w);
   dfree(ctx, z);
   dfree(ctx, r);
@@ -556,6 +585,9 @@ constexpr bool kDcgs2 = GPUFEM_GMRES_DCGS2;
 // each restart cycle runs until its residual estimate falls below this
 // fraction of rtol ||b|| (see the comment at its use)
 constexpr double kCycleTarget = GPUFEM_GMRES_CYCLE_TARGET;
+// sweep depth (L + U dependency levels) below which Arnoldi steps are issued
+// one ahead of the convergence test (see gmres_solve)
+constexpr int kLookaheadLevels = 128;
 
 int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
                 int max_iters, int stall_cycles, double rtol, int prec,
@@ -565,6 +597,22 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
   hipStream_t s = ctx->stream;
   const size_t N = W.N;
   const int m = W.m;
+  // the preconditioned basis, when the work space keeps it (FP64 and FP32
+  // preconditioner inside FP64 GMRES)
+  const bool keep_z = W.Z != nullptr && !ir;
+  // Look-ahead: Arnoldi step j+1 is issued before the host reads step j's
+  // residual estimate, hiding the host round trip (tens of microseconds)
+  // between steps; a step issued past the stopping point is a wasted
+  // preconditioner application.  With deep sweeps (a step of milliseconds)
+  // the waste costs more than the round trips, so look-ahead is used only
+  // for shallow sweeps (GPUFEM_GMRES_LOOKAHEAD=0/1 overrides).
+  bool ahead = true;
+  {
+    int nl = 0, nu = 0;
+    if (op.ilu && gf_ilu_info(op.ilu, nullptr, &nl, &nu) == GF_OK)
+      ahead = nl + nu < kLookaheadLevels;
+    if (const char* e = getenv("GPUFEM_GMRES_LOOKAHEAD")) ahead = atoi(e) != 0;
+  }
   // ||b||
   GF_TRY(W.dots(b, N, 1, b, W.scal, 2));
   GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.scal, sizeof(double),
@@ -656,7 +704,9 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
           const int js = issued;
           double* vj = W.V + (size_t)js * N;
           if (js < limit) {
-            GF_TRY(op.apply_prec_a(vj, W.w, prec));  // w = A M^-1 v_j
+            // w = A M^-1 v_j (M^-1 v_j kept in Z_j when Z is kept)
+            GF_TRY(op.apply_prec_a(vj, W.w, prec,
+                                   keep_z ? W.Z + (size_t)js * N : nullptr));
             const int nv = js + 1, nout = 2 * nv + 1;
             hipLaunchKernelGGL(multi_dot2_partial,
                                dim3(kDotBlocks, div_up(nv, kDotChunk)),
@@ -667,8 +717,12 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
                                0, (const int*)nullptr);
             hipLaunchKernelGGL(dcgs2_coeffs, dim3(1), dim3(256), shm, s, js,
                                m, W.dd, W.hc, W.rho, W.state, W.co, W.scal, 0);
-            hipLaunchKernelGGL(update2, grid1(N), dim3(256), 0, s, N, js, m,
-                               W.V, N, W.w, W.co);
+            if (keep_z)
+              hipLaunchKernelGGL(update2z, grid1(N), dim3(256), 0, s, N, js,
+                                 m, W.V, N, W.w, W.co, W.Z);
+            else
+              hipLaunchKernelGGL(update2, grid1(N), dim3(256), 0, s, N, js, m,
+                                 W.V, N, W.w, W.co);
           } else {
             GF_TRY(W.dots(W.V, N, js + 1, vj, W.dd, 0));
             hipLaunchKernelGGL(dcgs2_coeffs, dim3(1), dim3(256), shm, s, js,
@@ -683,8 +737,10 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
           }
           issued++;
         }
-        // iteration count j + 1 is reported by step j + 1
-        if (issued > j + 1 && (issued - (j + 1) >= 2 || issued > limit)) {
+        // iteration count j + 1 is reported by step j + 1; it is read once
+        // step j + 2 is queued when look-ahead is on, at once otherwise
+        if (issued > j + 1 &&
+            (!ahead || issued - (j + 1) >= 2 || issued > limit)) {
           const int r = j + 1;
           GF_HIP(ctx, hipEventSynchronize(W.ev[r % 2]));
           const double est = W.host_scal[2 * (r % 2)];
@@ -744,9 +800,15 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
     hipLaunchKernelGGL(solve_upper, dim3(1), dim3(1024),
                        sizeof(double) * (size_t)(j > 0 ? j : 1), s, j, m,
                        W.state, W.y);
-    hipLaunchKernelGGL(lincomb, grid1(N), dim3(256), 0, s, N, j, W.V, N, W.y,
-                       W.tmp);
-    GF_TRY(op.apply_prec(W.tmp, W.z, prec));
+    if (keep_z && kDcgs2) {
+      // x += Z y: the preconditioned basis is at hand
+      hipLaunchKernelGGL(lincomb, grid1(N), dim3(256), 0, s, N, j, W.Z, N,
+                         W.y, W.z);
+    } else {
+      hipLaunchKernelGGL(lincomb, grid1(N), dim3(256), 0, s, N, j, W.V, N,
+                         W.y, W.tmp);
+      GF_TRY(op.apply_prec(W.tmp, W.z, prec));
+    }
     hipLaunchKernelGGL(axpy_k, grid1(N), dim3(256), 0, s, N, 1.0, W.z, x);
     GF_TRY(check_launch(ctx, "gmres update"));
   }
@@ -776,19 +838,21 @@ int Operator::apply_prec(const double* in, double* out, int prec) const {
   return check_launch(ctx, "apply_prec fp32");
 }
 
-int Operator::apply_prec_a(const double* in, double* out, int prec) const {
+int Operator::apply_prec_a(const double* in, double* out, int prec,
+                           double* z_out) const {
   const size_t N = 6 * (size_t)n;
+  double* zt = z_out ? z_out : work64b;
   if (prec == 0) {
-    GF_TRY(gf_bsr_ilu_apply_f64(ilu, mlu, in, work64, work64b));
-    return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, work64b, out);
+    GF_TRY(gf_bsr_ilu_apply_f64(ilu, mlu, in, work64, zt));
+    return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, zt, out);
   }
   hipLaunchKernelGGL(to_f32, grid1(N), dim3(256), 0, ctx->stream, N, in, t32a);
   GF_TRY(gf_bsr_ilu_apply_f32(ilu, mlu32, t32a, t32b, t32c));
   if (prec == 2) {
     hipLaunchKernelGGL(to_f64, grid1(N), dim3(256), 0, ctx->stream, N, t32c,
-                       work64b);
+                       zt);
     GF_TRY(check_launch(ctx, "apply_prec_a fp32 ilu"));
-    return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, work64b, out);
+    return gf_bsr_spmv_f64(ctx, n, krowp, kcols, kvals, zt, out);
   }
   GF_TRY(gf_bsr_spmv_f32(ctx, n, krowp, kcols, kvals32, t32c, t32a));
   hipLaunchKernelGGL(to_f64, grid1(N), dim3(256), 0, ctx->stream, N, t32a, out);
@@ -824,6 +888,7 @@ struct KrySlot {
   const void* x = nullptr;  // preconditioned vector (sweep output, T)
   void* b = nullptr;        // sweep right-hand side (T)
   void* tmp = nullptr;      // L sweep output (T)
+  double* Z = nullptr;      // preconditioned basis (null: not kept)
 };
 
 // Arnoldi state of one solve.  mode (set by batch_begin for the step):
@@ -972,6 +1037,8 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// V (and Z, from the sweep output x, when kept) as update2 / update2z
+template <typename T>
 __global__ void batch_update2(int ns, long long ntot, const long long* off,
                               const KrySlot* ks, const SlotState* st) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -979,8 +1046,12 @@ __global__ void batch_update2(int ns, long long ntot, const long long* off,
   const int s = slot_of(off, ns, g);
   if (st[s].mode != 1) return;
   const KrySlot& k = ks[s];
-  update2_body((size_t)(g - off[s]), st[s].j, k.m, k.V, (size_t)k.N, k.w,
-               k.co);
+  const size_t t = (size_t)(g - off[s]);
+  const int j = st[s].j;
+  if (k.Z)
+    update_z_body(t, j, k.m, (double)reinterpret_cast<const T*>(k.x)[t], k.Z,
+                  (size_t)k.N, k.co);
+  update2_body(t, j, k.m, k.V, (size_t)k.N, k.w, k.co);
 }
 
 // one solve: b = (T) u, tmp = sentinel
@@ -1148,6 +1219,7 @@ int BatchEngine::rebuild() {
     k.co = p->kw.co;
     k.dd = p->kw.dd;
     k.partial = p->kw.partial;
+    k.Z = p->kw.Z;
     if (fp64()) {
       k.x = p->kw.z;
       k.b = p->work64;
@@ -1278,8 +1350,12 @@ int BatchEngine::issue_steps() {
                        d_state);
     hipLaunchKernelGGL(batch_coeffs, dim3(ns), dim3(256), shm, s, d_kry,
                        d_state);
-    hipLaunchKernelGGL(batch_update2, g1, dim3(256), 0, s, ns, ntot, d_voff,
-                       d_kry, d_state);
+    if (fp64())
+      hipLaunchKernelGGL(batch_update2<double>, g1, dim3(256), 0, s, ns, ntot,
+                         d_voff, d_kry, d_state);
+    else
+      hipLaunchKernelGGL(batch_update2<float>, g1, dim3(256), 0, s, ns, ntot,
+                         d_voff, d_kry, d_state);
     hipLaunchKernelGGL(batch_end, dim3(div_up(ns, 64)), dim3(64), 0, s, ns,
                        d_state);
   }
@@ -1412,7 +1488,8 @@ int BatchEngine::cycle_decide(Solve* v) {
   return check_launch(ctx, "batched cycle start");
 }
 
-// x += M^-1 V y over the cycle's iterations, then the next residual
+// x += M^-1 V y over the cycle's iterations (Z y when the preconditioned
+// basis is kept), then the next residual
 int BatchEngine::cycle_end(Solve* v) {
   gf_problem p = v->p;
   hipStream_t s = ctx->stream;
@@ -1422,6 +1499,14 @@ int BatchEngine::cycle_end(Solve* v) {
   hipLaunchKernelGGL(solve_upper, dim3(1), dim3(1024),
                      sizeof(double) * (size_t)(j > 0 ? j : 1), s, j, v->m,
                      W.state, W.y);
+  if (W.Z) {
+    hipLaunchKernelGGL(lincomb, grid1(v->N), dim3(256), 0, s, v->N, j, W.Z,
+                       v->N, W.y, W.tmp);
+    hipLaunchKernelGGL(axpy_from<double>, grid1(v->N), dim3(256), 0, s, v->N,
+                       (const double*)W.tmp, p->sol);
+    GF_TRY(check_launch(ctx, "batched cycle end"));
+    return residual_issue(v);
+  }
   hipLaunchKernelGGL(lincomb, grid1(v->N), dim3(256), 0, s, v->N, j, W.V,
                      v->N, W.y, W.tmp);
   const unsigned long long sent = sweep_sentinel_bits(fp64());
