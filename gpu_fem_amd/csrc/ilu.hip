@@ -959,6 +959,19 @@ __device__ __forceinline__ void store_value(T* p, T v) {
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// lane l's value, to every lane (v_readlane: a few cycles, where a
+// ds_bpermute shuffle takes an LDS round trip)
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float read_lane(float v, int l) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(unsigned, v), l));
+}
+
 template <typename T>
 __global__ void fill_sentinel_kernel(size_t n, T* x) {
   using B = typename Sentinel<T>::Bits;
@@ -1215,17 +1228,27 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
 #pragma unroll
       for (int u = 0; u < kS; u++) rs += __shfl(acc, g0 + 6 * u + (gl % 6), 64);
       s -= (gl < 6) ? rs : T(0);
+      // the diagonal block solve is on the critical path of every level:
+      // with one row per wavefront the source lane is uniform and
+      // v_readlane broadcasts it (400x400 plate, AMD + ILU(96): L+U pair
+      // 7.75 -> 7.34 ms against ds_bpermute shuffles)
+      auto bcast = [&](T v, int l) {
+        if constexpr (kRows == 1)
+          return read_lane(v, l);
+        else
+          return __shfl(v, g0 + l, 64);
+      };
       if (kLower) {
 #pragma unroll
         for (int mm = 0; mm < 5; mm++) {
-          const T ym = __shfl(s, g0 + mm, 64);
+          const T ym = bcast(s, mm);
           if (gl < 6 && gl > mm) s -= drow[mm] * ym;
         }
       } else {
 #pragma unroll
         for (int mm = 5; mm >= 0; mm--) {
           if (gl == mm) s *= dinv;
-          const T ym = __shfl(s, g0 + mm, 64);
+          const T ym = bcast(s, mm);
           if (gl < 6 && gl < mm) s -= drow[mm] * ym;
         }
       }
@@ -1233,7 +1256,7 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
       if (kUseCache) {
         cid = i;
 #pragma unroll
-        for (int c = 0; c < 6; c++) cx[c] = __shfl(s, c, 64);
+        for (int c = 0; c < 6; c++) cx[c] = read_lane(s, c);
       }
     }
     tn = __shfl(tn, 0, 64);
