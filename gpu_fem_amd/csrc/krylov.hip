@@ -342,6 +342,9 @@ __device__ __forceinline__ void dcgs2_coeffs_body(
   double* g = sn + m;
   for (int i = tid; i < j; i += 256) ls[i] = d[i];
   double alpha = 1.0;
+  // finalize 2: the estimate for j iterations only, nothing is written but
+  // out[] (the same column finalised with finalize 1 gives the same value)
+  const bool commit = finalize != 2;
   if (j > 0) {
     double ss = 0.0;
     for (int i = tid; i < j; i += 256) ss += d[i] * d[i];
@@ -353,7 +356,7 @@ __device__ __forceinline__ void dcgs2_coeffs_body(
     double* hc = Hc + (size_t)(j - 1) * ldh;
     for (int i = tid; i < j; i += 256) {
       const double h = hc[i] + r * ls[i];
-      hc[i] = h;
+      if (commit) hc[i] = h;
       col[i] = h;
     }
     for (int i = tid; i < j - 1; i += 256) {
@@ -361,7 +364,7 @@ __device__ __forceinline__ void dcgs2_coeffs_body(
       lsn[i] = sn[i];
     }
     if (tid == 0) {
-      hc[j] = r * alpha;
+      if (commit) hc[j] = r * alpha;
       col[j] = r * alpha;
     }
     __syncthreads();
@@ -376,17 +379,21 @@ __device__ __forceinline__ void dcgs2_coeffs_body(
       const double den = sqrt(a * a + c * c);
       const double cj = den > 0.0 ? a / den : 1.0;
       const double sj = den > 0.0 ? c / den : 0.0;
-      cs[jc] = cj;
-      sn[jc] = sj;
       col[jc] = cj * a + sj * c;
       col[jc + 1] = 0.0;
-      g[jc + 1] = -sj * g[jc];
-      g[jc] = cj * g[jc];
-      out[0] = fabs(g[jc + 1]);
+      const double gn = -sj * g[jc];
+      out[0] = fabs(gn);
       out[1] = r * alpha;
+      if (commit) {
+        cs[jc] = cj;
+        sn[jc] = sj;
+        g[jc + 1] = gn;
+        g[jc] = cj * g[jc];
+      }
     }
     __syncthreads();
-    for (int i = tid; i <= j; i += 256) R[(size_t)i * m + (j - 1)] = col[i];
+    if (commit)
+      for (int i = tid; i <= j; i += 256) R[(size_t)i * m + (j - 1)] = col[i];
   }
   if (finalize) return;
   const double inva = alpha > 0.0 ? 1.0 / alpha : 0.0;
@@ -588,6 +595,9 @@ constexpr double kCycleTarget = GPUFEM_GMRES_CYCLE_TARGET;
 // sweep depth (L + U dependency levels) below which Arnoldi steps are issued
 // one ahead of the convergence test (see gmres_solve)
 constexpr int kLookaheadLevels = 128;
+// an estimate is formed ahead of step j when the rate so far predicts it at
+// or below this multiple of the cycle's target
+constexpr double kPeekMargin = 4.0;
 
 int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
                 int max_iters, int stall_cycles, double rtol, int prec,
@@ -613,6 +623,10 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
       ahead = nl + nu < kLookaheadLevels;
     if (const char* e = getenv("GPUFEM_GMRES_LOOKAHEAD")) ahead = atoi(e) != 0;
   }
+  static const bool peek_on = [] {
+    const char* e = getenv("GPUFEM_GMRES_PEEK");
+    return !(e && atoi(e) == 0);
+  }();
   // ||b||
   GF_TRY(W.dots(b, N, 1, b, W.scal, 2));
   GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.scal, sizeof(double),
@@ -699,10 +713,39 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
       // Step js reports the estimate after js iterations (js >= 1), read
       // once step js+1 is queued.
       const size_t shm = sizeof(double) * (4 * (size_t)m + 2);
+      double e1 = 0.0, e2 = 0.0;  // the last two estimates of the cycle
       while (!stop) {
         if (issued <= limit) {
           const int js = issued;
           double* vj = W.V + (size_t)js * N;
+          // Without look-ahead, step js is issued once the estimate for js - 1
+          // iterations is known.  When the convergence rate so far predicts
+          // that js iterations meet the target, the estimate for js
+          // iterations is formed first from the dot products with v_js alone
+          // (no preconditioner application): if it meets the target, column
+          // js - 1 is finalised from the same products and the cycle ends
+          // without the sweeps of step js.
+          if (peek_on && !ahead && js >= 2 && js < limit && e2 > 0.0 &&
+              e1 * (e1 / e2) <= kPeekMargin * tol_abs) {
+            GF_TRY(W.dots(W.V, N, js + 1, vj, W.dd, 0));
+            hipLaunchKernelGGL(dcgs2_coeffs, dim3(1), dim3(256), shm, s, js,
+                               m, W.dd, W.hc, W.rho, W.state, W.co, W.scal, 2);
+            GF_HIP(ctx, hipMemcpyAsync(W.host_scal + 4, W.scal,
+                                       2 * sizeof(double),
+                                       hipMemcpyDeviceToHost, s));
+            GF_HIP(ctx, hipStreamSynchronize(s));
+            const double est = W.host_scal[4], hn = W.host_scal[5];
+            if (!std::isfinite(est) || est <= tol_abs || hn == 0.0) {
+              hipLaunchKernelGGL(dcgs2_coeffs, dim3(1), dim3(256), shm, s, js,
+                                 m, W.dd, W.hc, W.rho, W.state, W.co, W.scal,
+                                 1);
+              GF_TRY(check_launch(ctx, "arnoldi finalisation"));
+              last_est = est;
+              j = js;
+              stop = true;
+              break;
+            }
+          }
           if (js < limit) {
             // w = A M^-1 v_j (M^-1 v_j kept in Z_j when Z is kept)
             GF_TRY(op.apply_prec_a(vj, W.w, prec,
@@ -746,6 +789,8 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
           const double est = W.host_scal[2 * (r % 2)];
           const double hn = W.host_scal[2 * (r % 2) + 1];
           last_est = est;
+          e2 = e1;
+          e1 = est;
           j = r;
           if (!std::isfinite(est) || est <= tol_abs || hn == 0.0 || j == limit)
             stop = true;
