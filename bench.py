@@ -416,6 +416,22 @@ def main():
     sweep_bytes_launch = (lb + ub) / 2
     sweep_gbs = sweep_bytes_launch / sweep_launch_s / 1e9
     info_lv = ilu.info()
+    # the FP32 sweeps of the mixed paths, on the FP64 factors rounded to FP32
+    # (a sweep's traffic and dependency latency do not depend on the values)
+    mv32, b32 = mv.float(), b.float()
+    tmp32, xo32 = torch.zeros_like(b32), torch.zeros_like(b32)
+    ilu.apply(mv32, b32, tmp32, xo32)  # warm
+    ev.record(e0, stream)
+    for _ in range(nrep):
+        G.lib().gf_bsr_ilu_apply_f32(ilu.h, G.ptr(mv32), G.ptr(b32),
+                                     G.ptr(tmp32), G.ptr(xo32))
+    ev.record(e1, stream)
+    pair32_ms = ev.elapsed_ms(e0, e1) / nrep
+    ctx.check(G.lib().gf_ilu_check(ilu.h))
+    lb32, ub32 = sweep_bytes(mrp, mcp, 4)
+    sweep32_launch_s = pair32_ms * 1e-3 / 2
+    sweep32_bytes_launch = (lb32 + ub32) / 2
+    sweep32_gbs = sweep32_bytes_launch / sweep32_launch_s / 1e9
 
     # ---- assembly kernels (Jacobian, colour-ordered scatter as in the
     # problem) in isolation
@@ -511,7 +527,8 @@ def main():
         "latency_ms": {"p50": r["p50_ms"], "p99": r["p99_ms"]},
         "paths": {k: {kk: vv for kk, vv in v.items() if kk != "x_host"}
                   for k, v in results.items()},
-        "roofline": {"bound": "hbm", "kernel": "trsv_syncfree_kernel (L and U)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "trsv_syncfree_kernel<double> (L and U)",
                      "achieved": sweep_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sweep_gbs / HBM_PEAK_GBS,
                      "traffic": (pmc["sweep_bytes_per_launch"] if pmc
@@ -524,6 +541,13 @@ def main():
                      "bytes_per_launch": sweep_bytes_launch,
                      "launch_us": sweep_launch_s * 1e6,
                      "levels": [info_lv["levels_lower"], info_lv["levels_upper"]]},
+        # the dominant kernel of the mixed paths (FP32 factors and vectors)
+        "roofline_fp32_sweep": {
+            "bound": "hbm", "kernel": "trsv_syncfree_kernel<float> (L and U)",
+            "achieved": sweep32_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": sweep32_gbs / HBM_PEAK_GBS,
+            "bytes_per_launch": sweep32_bytes_launch,
+            "launch_us": sweep32_launch_s * 1e6},
         "assembly_roofline": {"bound": "hbm",
                               "kernel": f"jacobian_mfma_kernel, {ncolors} "
                                         "colour launches",
