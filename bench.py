@@ -124,6 +124,10 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     # short kernel-only run for profilers (rocprofv3 --pmc / --kernel-trace)
     ap.add_argument("--profile-kernels", action="store_true")
+    ap.add_argument("--residual-floor", action="store_true",
+                    help="after the run, compare the FP64 path's true "
+                         "residual with the residual that rounding its "
+                         "solution to FP64 causes")
     return ap.parse_args()
 
 
@@ -247,6 +251,59 @@ def live_pmc_traffic(args):
             "numeric_ilu": bytes_of("ilu_row_", 1 << 30),
             "calibration": {"fetch_kib_per_gib": max(copy_f),
                             "write_kib_per_gib": max(copy_w)}}
+
+
+def residual_floor(ctx, p, mesh, x, hip_memcpy):
+    """The true relative residual of the FP64 path against what FP64 storage
+    of the solution allows: eps |K| |x| (a bound on the residual that
+    rounding x causes) and the residual change when every entry of x moves
+    by a half unit in the last place with random sign.  A correctly rounded
+    solution has a residual of the order of the latter."""
+    import torch
+
+    import gpu_fem_amd as G
+
+    L = G.lib()
+    d = p.device_arrays()
+    nn, nk, _ = p.sizes()
+    perm = p.perm()
+
+    def permute(v):
+        out = np.empty_like(v)
+        out.reshape(-1, 6)[perm] = v.reshape(-1, 6)
+        return out
+
+    loads = mesh.loads.copy()
+    loads[mesh.bcs] = 0.0
+    kv = torch.empty(36 * nk, dtype=torch.float64, device="cuda")
+    ctx.sync()
+    assert hip_memcpy(ctypes.c_void_p(kv.data_ptr()),
+                      ctypes.c_void_p(d["kvals"]), kv.numel() * 8, 3) == 0
+    db = torch.from_numpy(permute(loads)).cuda()
+    xp = permute(x)
+    rowp, cols = ctypes.c_void_p(d["krowp"]), ctypes.c_void_p(d["kcols"])
+
+    def resid(xv):
+        dx = torch.from_numpy(xv).cuda()
+        r = torch.zeros_like(db)
+        ctx.check(L.gf_bsr_resid_comp_f64(ctx.h, nn, rowp, cols, G.ptr(kv),
+                                          G.ptr(dx), G.ptr(db), G.ptr(r)))
+        ctx.sync()
+        return r
+
+    half_ulp = np.finfo(np.float64).eps / 2
+    kabs, ax = kv.abs(), torch.from_numpy(np.abs(xp)).cuda()
+    y = torch.zeros_like(db)
+    ctx.check(L.gf_bsr_spmv_f64(ctx.h, nn, rowp, cols, G.ptr(kabs), G.ptr(ax),
+                                G.ptr(y)))
+    ctx.sync()
+    bn = db.norm().item()
+    r0 = resid(xp)
+    rng = np.random.default_rng(0)
+    r1 = resid(xp * (1.0 + half_ulp * rng.choice([-1.0, 1.0], xp.size)))
+    return {"true_rel_resid": r0.norm().item() / bn,
+            "eps_absK_absx_over_b": half_ulp * y.norm().item() / bn,
+            "half_ulp_random_change": (r1 - r0).norm().item() / bn}
 
 
 def sweep_bytes(mrp, mcp, w):
@@ -432,6 +489,11 @@ def main():
     sweep32_launch_s = pair32_ms * 1e-3 / 2
     sweep32_bytes_launch = (lb32 + ub32) / 2
     sweep32_gbs = sweep32_bytes_launch / sweep32_launch_s / 1e9
+    floor = None
+    if args.residual_floor and "fp64" in results:
+        floor = residual_floor(ctx, problems["fp64"], mesh,
+                               results["fp64"]["x_host"], hip_memcpy)
+        print(f"[bench] residual floor: {floor}", file=sys.stderr, flush=True)
 
     # ---- assembly kernels (Jacobian, colour-ordered scatter as in the
     # problem) in isolation
@@ -541,6 +603,7 @@ def main():
                      "bytes_per_launch": sweep_bytes_launch,
                      "launch_us": sweep_launch_s * 1e6,
                      "levels": [info_lv["levels_lower"], info_lv["levels_upper"]]},
+        "residual_floor": floor,
         # the dominant kernel of the mixed paths (FP32 factors and vectors)
         "roofline_fp32_sweep": {
             "bound": "hbm", "kernel": "trsv_syncfree_kernel<float> (L and U)",
