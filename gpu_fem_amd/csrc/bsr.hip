@@ -56,32 +56,54 @@ __global__ void bc_vec_kernel(int nbcs, const int* __restrict__ bcs,
   vec[idx] = 0.0;
 }
 
-// one wave per target row; lanes stride over the row's 36*nblocks values
+// One wave per target row, in chunks of 64 target blocks: lane l finds the
+// source block of target block l of the chunk (binary search in the short
+// source row, once per block), the indices go through LDS, and the chunk's
+// 36 * 64 values are written with consecutive lanes on consecutive values
+// (most target blocks are fill: zeros).
 __global__ void __launch_bounds__(256)
     copy_pattern_kernel(int nnodes, const int* __restrict__ rowp,
                         const int* __restrict__ cols,
                         const double* __restrict__ vals,
                         const int* __restrict__ trowp,
                         const int* __restrict__ tcols, double* tvals) {
-  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  __shared__ int s_src[4][64];
+  const int w = threadIdx.x / 64;
+  const int row = blockIdx.x * 4 + w;
   const int lane = threadIdx.x % 64;
   if (row >= nnodes) return;
   const int t0 = trowp[row], t1 = trowp[row + 1];
   const int s0 = rowp[row], s1 = rowp[row + 1];
-  const int nval = 36 * (t1 - t0);
-  for (int v = lane; v < nval; v += 64) {
-    const int tb = t0 + v / 36, e = v % 36;
-    const int col = tcols[tb];
-    int lo = s0, hi = s1;  // lower_bound of col in the source row
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (cols[mid] < col)
-        lo = mid + 1;
-      else
-        hi = mid;
+  for (int c0 = t0; c0 < t1; c0 += 64) {
+    const int nb = min(64, t1 - c0);
+    int src = -1;
+    if (lane < nb) {
+      const int col = tcols[c0 + lane];
+      int lo = s0, hi = s1;  // lower_bound of col in the source row
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cols[mid] < col)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      src = (lo < s1 && cols[lo] == col) ? lo : -1;
     }
-    const double x = (lo < s1 && cols[lo] == col) ? vals[36 * (size_t)lo + e] : 0.0;
-    tvals[36 * (size_t)t0 + v] = x;
+    s_src[w][lane] = src;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double* dst = tvals + 36 * (size_t)c0;
+    const int nval = 36 * nb;
+#pragma unroll 4
+    for (int v = lane; v < nval; v += 64) {
+      const int sb = s_src[w][v / 36];
+      dst[v] = sb >= 0 ? vals[36 * (size_t)sb + v % 36] : 0.0;
+    }
+    // the next chunk's indices overwrite s_src
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
