@@ -623,10 +623,8 @@ int gmres_solve(KrylovWork& W, const Operator& op, const double* b, double* x,
       ahead = nl + nu < kLookaheadLevels;
     if (const char* e = getenv("GPUFEM_GMRES_LOOKAHEAD")) ahead = atoi(e) != 0;
   }
-  static const bool peek_on = [] {
-    const char* e = getenv("GPUFEM_GMRES_PEEK");
-    return !(e && atoi(e) == 0);
-  }();
+  bool peek_on = true;
+  if (const char* e = getenv("GPUFEM_GMRES_PEEK")) peek_on = atoi(e) != 0;
   // ||b||
   GF_TRY(W.dots(b, N, 1, b, W.scal, 2));
   GF_HIP(ctx, hipMemcpyAsync(W.host_scal, W.scal, sizeof(double),

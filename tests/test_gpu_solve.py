@@ -181,3 +181,39 @@ def test_stall_cycles_stop_a_stagnating_solve(gpu_ctx):
     # reference's does; with it, it stops after a handful of cycles
     assert its[0][0] == 1500 and "iteration limit" in its[0][2]
     assert its[3][0] < 1500 and its[3][1] <= 20 and "stagnated" in its[3][2]
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_gmres_scheduling_switches(gpu_ctx, precision, monkeypatch):
+    """The Arnoldi look-ahead and the estimate formed before a cycle's likely
+    last step change when work is issued, not what is computed: look-ahead
+    on and off give the same bits; the estimate-first step (with look-ahead
+    off, where it applies) gives the same iteration count and the same
+    solution to rounding (its column j - 1 is finalised from dot products
+    summed by another kernel).  Both solve to the oracle's 1e-8."""
+    say(f"[gpu] GMRES scheduling switches precision={precision}")
+    name = "cylinder_amd_ilu2"
+    _, build, reorder, k, restart, rtol = CASE[name]
+    mesh = build()
+
+    def solve(lookahead, peek):
+        monkeypatch.setenv("GPUFEM_GMRES_LOOKAHEAD", str(lookahead))
+        monkeypatch.setenv("GPUFEM_GMRES_PEEK", str(peek))
+        p = G.Problem(gpu_ctx, mesh, reorder=reorder, lev_fill=k,
+                      restart=restart,
+                      rtol=rtol, max_iters=40000, precision=precision)
+        p.assemble()
+        p.factor()
+        x, info = p.solve()
+        p.close()
+        return x, info.iterations, info.outer
+
+    x_a, it_a, out_a = solve(1, 0)
+    x_b, it_b, out_b = solve(0, 0)
+    x_c, it_c, out_c = solve(0, 1)
+    assert it_a > 0
+    assert np.array_equal(x_a, x_b) and (it_a, out_a) == (it_b, out_b)
+    assert (it_c, out_c) == (it_b, out_b)
+    assert np.linalg.norm(x_c - x_b) <= 1e-10 * np.linalg.norm(x_b)
+    _, _, x_o, _ = oracle_case(name)
+    assert np.linalg.norm(x_c - x_o) <= 1e-8 * np.linalg.norm(x_o)
