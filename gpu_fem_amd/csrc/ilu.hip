@@ -628,8 +628,19 @@ __global__ void __launch_bounds__(kIluThreads)
 #endif
 constexpr int kIluPairs = GPUFEM_ILU_PAIRS;  // (block, column) pairs of U_kj per thread
 constexpr int kIluCols = 4;   // prefetched column indices per staging thread
-template <typename T, bool kFlow>
-__global__ void __launch_bounds__(kIluThreads)
+// Short rows (levels whose rows, and so their pivot rows' U parts, are
+// short) run in 256-thread workgroups: the kernel's registers allow 12
+// wavefronts a CU, one 512-thread row or three 256-thread ones, and a short
+// row spends its time in per-row latency (ticket, loads of its pattern and
+// pivot bounds, diagonal factorisation, write-back and release), not in its
+// few pivot updates.  400x400 plate, AMD + ILU(96), FP64 factorisation:
+// 135 ms with 512 threads throughout, 129 ms with rows up to 128 blocks in
+// 256-thread workgroups, 123 ms up to 256 or 512 blocks.  (The staging
+// threads hold 4 * (256 - 6) column indices: pivot U parts up to 1,000.)
+constexpr int kIluShortThreads = 256;
+constexpr int kIluShortRow = 256;  // longest row (blocks) taken that way
+template <typename T, bool kFlow, int kThreads = kIluThreads>
+__global__ void __launch_bounds__(kThreads)
     ilu_row_lds_kernel(const int* __restrict__ level_rows, int nrows,
                        const int* __restrict__ rowp,
                        const int* __restrict__ cols,
@@ -1581,8 +1592,11 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   static std::once_flag once[2];
   const int ti = sizeof(T) == 8 ? 0 : 1;
   std::call_once(once[ti], [&] {
-    const void* fns[2] = {(const void*)gf::ilu_row_lds_kernel<T, false>,
-                          (const void*)gf::ilu_row_lds_kernel<T, true>};
+    const void* fns[4] = {
+        (const void*)gf::ilu_row_lds_kernel<T, false>,
+        (const void*)gf::ilu_row_lds_kernel<T, true>,
+        (const void*)gf::ilu_row_lds_kernel<T, true, gf::kIluShortThreads>,
+        (const void*)gf::ilu_row_lds_kernel<T, false, gf::kIluShortThreads>};
     hipDeviceProp_t prop{};
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return;
     const size_t total = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
@@ -1680,6 +1694,7 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
   if (const char* cenv = getenv("GPUFEM_ILU_LDS_CAP"))
     cap = std::min<size_t>(cap, (size_t)std::max(0L, atol(cenv)));
   const char* senv = getenv("GPUFEM_ILU_SCHEDULE");
+  const bool short_off = getenv("GPUFEM_ILU_SHORT_OFF") != nullptr;
   const bool by_level = senv && std::string(senv) == "level";
   const int nl = (int)h->lptr.size() - 1;
   // row values, column indices and pivot bounds of the level's longest
@@ -1717,6 +1732,10 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
     int first_global = 0;
     while (first_global < nl && !no_lds && dyn_of(first_global) <= cap)
       first_global++;
+    // longest row up to each level: it bounds the U part of every pivot row
+    // of the rows up to there
+    std::vector<int> pmax(nl + 1, 0);
+    for (int l = 0; l < nl; l++) pmax[l + 1] = std::max(pmax[l], h->lmaxlen[l]);
     int l0 = 0;
     while (l0 < nl) {
       const bool lds = l0 < first_global;
@@ -1737,13 +1756,27 @@ int ilu_factor(gf_ilu h, T* vals, int* zero_pivot) {
       f.done = h->done;
       f.epoch = h->epoch;
       f.err = h->err + 1;
-      if (lds)
+      if (lds && pmax[l1] <= gf::kIluShortRow && !short_off) {
+        // short rows: 256-thread workgroups, position lists as long as the
+        // longest row so far
+        int lmax = 0;
+        for (int l = l0; l < l1; l++) lmax = std::max(lmax, h->lmaxlen[l]);
+        const size_t dyn_s =
+            (size_t)lmax * (36 * sizeof(T) + 3 * sizeof(int)) +
+            2 * sizeof(int) * (size_t)pmax[l1];
+        hipLaunchKernelGGL(
+            (gf::ilu_row_lds_kernel<T, true, gf::kIluShortThreads>),
+            dim3(cnt), dim3(gf::kIluShortThreads), dyn_s, ctx->stream,
+            h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals, h->pivot, f,
+            pmax[l1]);
+      } else if (lds) {
         hipLaunchKernelGGL((gf::ilu_row_lds_kernel<T, true>), dim3(cnt),
                            dim3(gf::kIluThreads), dyn, ctx->stream,
                            h->lrows + r0, cnt, h->rowp, h->cols, h->diag, vals,
                            h->pivot, f, maxlen);
-      else
+      } else {
         launch_global(true, h->lrows + r0, cnt, f);
+      }
       h->base += (unsigned long long)cnt;
       l0 = l1;
     }
