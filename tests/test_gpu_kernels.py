@@ -151,6 +151,25 @@ def test_bc_copy_permute_spmv(gpu_ctx):
     gpu_ctx.sync()
     np.testing.assert_array_equal(
         host(d_t), oracle.copy_values(s["krp"], s["kcp"], ref, s["mrp"], s["mcp"]))
+    # and into a complete-fill pattern whose rows (up to 161 blocks) take
+    # up to three chunks of 64 target blocks
+    m2 = M.plate(40, 40)
+    rp2, cp2 = oracle.node_graph(m2.conn, m2.nnodes)
+    krp2, kcp2 = oracle.permute_pattern(rp2, cp2, oracle.rcm(rp2, cp2))
+    frp, fcp, _ = oracle.iluk(krp2, kcp2, 1 << 29)
+    assert np.diff(frp).max() > 64
+    v2 = rng.standard_normal(36 * int(krp2[-1]))
+    d_f = torch.full((36 * int(frp[-1]),), -3.0, dtype=torch.float64,
+                     device="cuda")
+    d_krp2, d_kcp2, d_v2 = dev(krp2, np.int32), dev(kcp2, np.int32), dev(v2)
+    d_frp, d_fcp = dev(frp, np.int32), dev(fcp, np.int32)
+    gpu_ctx.check(L.gf_bsr_copy_to_pattern(gpu_ctx.h, m2.nnodes, G.ptr(d_krp2),
+                                           G.ptr(d_kcp2), G.ptr(d_v2),
+                                           G.ptr(d_frp), G.ptr(d_fcp),
+                                           G.ptr(d_f)))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(
+        host(d_f), oracle.copy_values(krp2, kcp2, v2, frp, fcp))
     # vector BCs and permutation
     v = rng.standard_normal(6 * n)
     d_vec = dev(v)
