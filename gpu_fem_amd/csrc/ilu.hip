@@ -907,12 +907,15 @@ __global__ void __launch_bounds__(256)
 constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 // Batches of 10 blocks of a pass whose factor values are loaded into
 // registers before the row waits; blocks beyond them are loaded after the
-// wait, on the row's critical path.  400x400 plate, AMD + ILU(16), L+U pair:
-// 3 -> 4.33 ms, 4 -> 4.09, 6 -> 3.71, 8 -> 3.48, 10 -> 3.36, 12 -> 3.29
-// (FP64: 256 VGPRs and a few AGPRs, one wavefront per SIMD, which still
-// holds the three wavefronts per CU a sweep launches; no scratch).
+// wait, on the row's critical path.  With passes of 128 blocks
+// (400x400 plate, AMD + ILU(16), L+U pair): 3 -> 4.33 ms, 4 -> 4.09,
+// 6 -> 3.71, 8 -> 3.48, 10 -> 3.36, 12 -> 3.29 (256 VGPRs).  At ILU(96)
+// (rows of hundreds of blocks, so most passes are full): 12 batches of a
+// 128-block pass 7.32 ms, 13 (the whole pass) 6.83 ms, passes of 64 blocks
+// with 7 batches (the whole pass, 178 VGPRs) 5.99 ms: no load is left
+// after the wait, and the wait gathers half as many dependencies.
 #ifndef GPUFEM_SWEEP_ITER
-#define GPUFEM_SWEEP_ITER 12
+#define GPUFEM_SWEEP_ITER 7
 #endif
 #ifndef GPUFEM_SWEEP_MINWAVES
 #define GPUFEM_SWEEP_MINWAVES 1
@@ -935,13 +938,15 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 // mean less polling and counter traffic; measured on the 400x400 plate
 // (AMD + ILU(12)): 16 -> 4.37 ms per L+U pair, 4 -> 3.62, 3 -> 3.51,
 // 2 -> 3.64, 1 -> 5.17; with 12 register batches (ILU(16)): 2 -> 3.55,
-// 3 -> 3.26, 4 -> 3.31, 5 -> 3.61
+// 3 -> 3.26, 4 -> 3.31, 5 -> 3.61; with 64-block passes (ILU(96)): 3 -> 5.99,
+// 4 -> 5.95, 6 -> 6.13, 8 -> 6.19
 #ifndef GPUFEM_SWEEP_WAVES_PER_CU
 #define GPUFEM_SWEEP_WAVES_PER_CU 3
 #endif
 constexpr int kSweepIter = GPUFEM_SWEEP_ITER;  // block batches in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
-constexpr int kGather = 128;              // dependency blocks per pass
+constexpr int kGather = 64;               // dependency blocks per pass
+static_assert(kGather % 64 == 0, "a pass spreads over whole wavefronts");
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
 // Sentinel values: signalling NaNs.  Arithmetic never produces a signalling
