@@ -946,6 +946,10 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 constexpr int kSweepIter = GPUFEM_SWEEP_ITER;  // block batches in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
 constexpr int kGather = 64;               // dependency blocks per pass
+// blocks next to the diagonal taken as a row's last pass: ILU(96) plate,
+// L+U pair 5.99 ms without, 5.90 / 5.82 / 5.84 / 5.87 / 5.89 ms with 4 / 6 /
+// 8 / 16 / 24
+constexpr int kNearPass = 8;
 static_assert(kGather % 64 == 0, "a pass spreads over whole wavefronts");
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
@@ -1143,20 +1147,33 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
 #endif
       T acc = T(0);
       const int nb = k1 - k0;
-      int npass = (nb + kGather - 1) / kGather;
+      // The row's latest dependency is usually its nearest row: the last
+      // block of an L row, the first of a U row.  The kNearPass blocks next
+      // to the diagonal form the last pass (of rows longer than that), so
+      // that little work is left once the latest dependency arrives; the
+      // rest go in passes of kGather blocks, U rows taking theirs last to
+      // first, so that the passes that wait longest come last and the
+      // others overlap with the wait.
+      const bool split = kNearPass > 0 && kRows == 1 && nb > kNearPass;
+      const int nfar = split ? nb - kNearPass : nb;
+      const int nfp = (nfar + kGather - 1) / kGather;
+      int npass = nfp + (split ? 1 : 0);
       // every row group of the wavefront runs the same number of passes
 #pragma unroll
       for (int o = G; o < 64; o <<= 1)
         npass = max(npass, __shfl_xor(npass, o, 64));
-      // the row's latest dependency is usually its nearest row: the last
-      // block of an L row, the first of a U row.  U rows take their passes
-      // last to first, so that the pass that waits longest comes last and
-      // the others overlap with the wait (rows of over kGather blocks: the
-      // chains of a complete factorisation)
       for (int pi = 0; pi < npass; pi++) {
-        const int ps = kLower ? pi : npass - 1 - pi;
-        const int g0b = k0 + ps * kGather;
-        const int ng = max(0, min(kGather, k1 - g0b));
+        int g0b, ng;
+        if (split && pi == npass - 1) {
+          g0b = kLower ? k1 - kNearPass : k0;
+          ng = kNearPass;
+        } else {
+          const int j = kLower ? pi : nfp - 1 - pi;
+          const int fb = kLower ? k0 : k0 + (split ? kNearPass : 0);
+          const int fe = kLower ? (split ? k1 - kNearPass : k1) : k1;
+          g0b = fb + j * kGather;
+          ng = max(0, min(kGather, fe - g0b));
+        }
         T a[kSweepIter][6];
 #pragma unroll
         for (int it = 0; it < kSweepIter; it++) {
