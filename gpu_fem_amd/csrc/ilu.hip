@@ -948,7 +948,8 @@ constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
 constexpr int kGather = 64;               // dependency blocks per pass
 // blocks next to the diagonal taken as a row's last pass: ILU(96) plate,
 // L+U pair 5.99 ms without, 5.90 / 5.82 / 5.84 / 5.87 / 5.89 ms with 4 / 6 /
-// 8 / 16 / 24
+// 8 / 16 / 24 in every row longer than that, 5.77 ms with 8 in the rows of
+// more than one pass (a row of at most kGather blocks keeps its one pass)
 constexpr int kNearPass = 8;
 static_assert(kGather % 64 == 0, "a pass spreads over whole wavefronts");
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
@@ -1148,13 +1149,13 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
       T acc = T(0);
       const int nb = k1 - k0;
       // The row's latest dependency is usually its nearest row: the last
-      // block of an L row, the first of a U row.  The kNearPass blocks next
-      // to the diagonal form the last pass (of rows longer than that), so
-      // that little work is left once the latest dependency arrives; the
+      // block of an L row, the first of a U row.  In rows of more than one
+      // pass the kNearPass blocks next to the diagonal form the last pass,
+      // so that little work is left once the latest dependency arrives; the
       // rest go in passes of kGather blocks, U rows taking theirs last to
       // first, so that the passes that wait longest come last and the
       // others overlap with the wait.
-      const bool split = kNearPass > 0 && kRows == 1 && nb > kNearPass;
+      const bool split = kNearPass > 0 && kRows == 1 && nb > kGather;
       const int nfar = split ? nb - kNearPass : nb;
       const int nfp = (nfar + kGather - 1) / kGather;
       int npass = nfp + (split ? 1 : 0);
