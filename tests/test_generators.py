@@ -1,5 +1,5 @@
-"""Generators of the workloads: the reference plate's
-simply supported BC pattern and point-sampled loads
+"""Generators of the workloads (plates, cylinders, wingboxes): the
+reference plate's simply supported BC pattern and point-sampled loads
 (examples/plate_demo/_plate_utils.h:51-82, :146-184, :186-212) as known
 answers derived by hand from the reference's loops, load resultants of the
 consistent loads against closed forms, and (seed, SHA-256) fixtures of the
