@@ -324,8 +324,8 @@ def sweep_bytes(mrp, mcp, w):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for "
-              f"--gpus {args.gpus}); the multi-GPU path is design-only",
+        print(f"bench.py measures one GPU only (asked for --gpus "
+              f"{args.gpus}); the multi-GPU path is design-only",
               file=sys.stderr)
         sys.exit(2)
     if args.serve:
