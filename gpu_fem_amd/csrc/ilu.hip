@@ -934,6 +934,11 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 #ifndef GPUFEM_SWEEP_BACKOFF
 #define GPUFEM_SWEEP_BACKOFF 2
 #endif
+// A waiting row polls only its pending entries, one poll in flight.  Two
+// interleaved streams of polls of all the row's entries, the second half a
+// round trip after the first, took the L+U pair of the 400x400 plate (AMD +
+// ILU(96)) from 5.69 to 8.8 ms: polling traffic delays the stores and loads
+// on the critical path more than the earlier detection gains.
 // wavefronts per compute unit for a sweep launch: fewer waiting wavefronts
 // mean less polling and counter traffic; measured on the 400x400 plate
 // (AMD + ILU(12)): 16 -> 4.37 ms per L+U pair, 4 -> 3.62, 3 -> 3.51,
@@ -1119,6 +1124,10 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
       for (int mm = 0; mm < 6; mm++)
         if (gl == mm) dd = drow[mm];
       const T dinv = (!kLower && live && gl < 6) ? T(1) / dd : T(0);
+      // the compiler would otherwise sink the division (a dozen dependent
+      // instructions) past the waits onto the critical path: 400x400 plate,
+      // AMD + ILU(96), L+U pair 5.78 -> 5.69 ms
+      asm volatile("" ::"v"(dinv));
       // Per pass of up to kGather blocks: the first kSweepIter*kS blocks'
       // values are loaded before anything waits; the pass's column indices
       // are read (lane q: blocks q, q+G, ...) and the dependencies' entries
