@@ -930,7 +930,8 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 #endif
 // longest sleep between polls, x 64 cycles: L+U pair on the 400x400 plate
 // (ILU(16), 4 batches) 16 -> 4.14 ms, 8 -> 4.09, 4 -> 4.06, 2 -> 4.06,
-// 1 -> 4.07; with 12 batches 8 -> 3.29, 2 -> 3.26
+// 1 -> 4.07; with 12 batches 8 -> 3.29, 2 -> 3.26; ILU(96), entries of a
+// dependency in consecutive lanes: 1 -> 5.29, 2 -> 5.26, 4 -> 5.25
 #ifndef GPUFEM_SWEEP_BACKOFF
 #define GPUFEM_SWEEP_BACKOFF 2
 #endif
@@ -944,9 +945,10 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 // (AMD + ILU(12)): 16 -> 4.37 ms per L+U pair, 4 -> 3.62, 3 -> 3.51,
 // 2 -> 3.64, 1 -> 5.17; with 12 register batches (ILU(16)): 2 -> 3.55,
 // 3 -> 3.26, 4 -> 3.31, 5 -> 3.61; with 64-block passes (ILU(96)): 3 -> 5.99,
-// 4 -> 5.95, 6 -> 6.13, 8 -> 6.19
+// 4 -> 5.95, 6 -> 6.13, 8 -> 6.19; with the entries of a dependency in
+// consecutive lanes (less memory traffic per wavefront): 3 -> 5.26, 4 -> 5.12
 #ifndef GPUFEM_SWEEP_WAVES_PER_CU
-#define GPUFEM_SWEEP_WAVES_PER_CU 3
+#define GPUFEM_SWEEP_WAVES_PER_CU 4
 #endif
 constexpr int kSweepIter = GPUFEM_SWEEP_ITER;  // block batches in registers
 constexpr int kSlots = 10;                // blocks per batch (6 lanes each)
@@ -954,8 +956,13 @@ constexpr int kGather = 64;               // dependency blocks per pass
 // blocks next to the diagonal taken as a row's last pass: ILU(96) plate,
 // L+U pair 5.99 ms without, 5.90 / 5.82 / 5.84 / 5.87 / 5.89 ms with 4 / 6 /
 // 8 / 16 / 24 in every row longer than that, 5.77 ms with 8 in the rows of
-// more than one pass (a row of at most kGather blocks keeps its one pass)
-constexpr int kNearPass = 8;
+// more than one pass (a row of at most kGather blocks keeps its one pass);
+// with the entries of a dependency in consecutive lanes 5 / 8 / 10: 5.27 /
+// 5.26 / 5.26 ms
+#ifndef GPUFEM_SWEEP_NEAR
+#define GPUFEM_SWEEP_NEAR 8
+#endif
+constexpr int kNearPass = GPUFEM_SWEEP_NEAR;
 static_assert(kGather % 64 == 0, "a pass spreads over whole wavefronts");
 constexpr unsigned kSpinLimit = 1u << 20; // bounded polls per batch
 
@@ -1193,36 +1200,50 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
 #pragma unroll
           for (int c = 0; c < 6; c++) a[it][c] = ok ? ap[c] : T(0);
         }
-        int jj[kH];
-        B xb[kH][6];
+        // The pass's dependency entries, kE per lane.  One row per
+        // wavefront: lane l holds entries e = l + 64 k of the pass's 6 ng
+        // (dependency e / 6, component e % 6), so the six entries of a
+        // dependency sit in consecutive lanes and each load instruction
+        // touches a sixth as many cache lines as with a dependency per lane;
+        // several rows per wavefront: lane gl holds the six entries of
+        // dependencies gl + G h.  400x400 plate, AMD + ILU(96), L+U pair:
+        // 5.69 ms with a dependency per lane, 5.26 ms with its entries in
+        // consecutive lanes (each poll of a waiting row is then one or two
+        // requests instead of six).
+        constexpr bool kT = kRows == 1;
+        constexpr int kE = 6 * kH;
+        int jq[kE], jc[kE], jj[kE];
+        B xb[kE];
 #pragma unroll
-        for (int h = 0; h < kH; h++) {
-          const int q = gl + G * h;
-          jj[h] = q < ng ? cols[g0b + q] : -1;
-#pragma unroll
-          for (int c = 0; c < 6; c++)
-            xb[h][c] = (jj[h] >= 0 && !cached(jj[h]))
-                           ? load_bits(x + 6 * (size_t)jj[h] + c)
-                           : B(0);
+        for (int k = 0; k < kE; k++) {
+          if constexpr (kT) {
+            jq[k] = (lane + 64 * k) / 6;
+            jc[k] = (lane + 64 * k) % 6;
+          } else {
+            jq[k] = gl + G * (k / 6);
+            jc[k] = k % 6;
+          }
+          jj[k] = jq[k] < ng ? cols[g0b + jq[k]] : -1;
         }
+#pragma unroll
+        for (int k = 0; k < kE; k++)
+          xb[k] = (jj[k] >= 0 && !cached(jj[k]))
+                      ? load_bits(x + 6 * (size_t)jj[k] + jc[k])
+                      : B(0);
         unsigned spins = 0;
         int nap = 1;
         for (;;) {
           bool pending = false;
 #pragma unroll
-          for (int h = 0; h < kH; h++)
-#pragma unroll
-            for (int c = 0; c < 6; c++)
-              pending |= xb[h][c] == Sentinel<T>::value;
+          for (int k = 0; k < kE; k++)
+            pending |= xb[k] == Sentinel<T>::value;
           if (!__any(pending)) break;
           if (++spins > kSpinLimit) {
             if (lane == 0)
               __hip_atomic_store(err, 1, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-            for (int h = 0; h < kH; h++)
-#pragma unroll
-              for (int c = 0; c < 6; c++) xb[h][c] = B(0);
+            for (int k = 0; k < kE; k++) xb[k] = B(0);
             break;
           }
           // back off while the dependencies are not ready: many waiting
@@ -1230,20 +1251,23 @@ __global__ void __launch_bounds__(kSweepThreads, GPUFEM_SWEEP_MINWAVES)
           for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(1);
           nap = nap * 2 > GPUFEM_SWEEP_BACKOFF ? GPUFEM_SWEEP_BACKOFF : nap * 2;
 #pragma unroll
-          for (int h = 0; h < kH; h++)
-#pragma unroll
-            for (int c = 0; c < 6; c++)
-              if (xb[h][c] == Sentinel<T>::value)
-                xb[h][c] = load_bits(x + 6 * (size_t)jj[h] + c);
+          for (int k = 0; k < kE; k++)
+            if (xb[k] == Sentinel<T>::value)
+              xb[k] = load_bits(x + 6 * (size_t)jj[k] + jc[k]);
         }
         // park the dependencies in this row group's LDS slice
 #pragma unroll
-        for (int h = 0; h < kH; h++) {
-          const int q = gl + G * h;
-          if (q < ng) {
+        for (int k = 0; k < kE; k++) {
+          if (jq[k] < ng) {
+            T v = __builtin_bit_cast(T, xb[k]);
+            if (cached(jj[k])) {
+              // cx[jc[k]] without a dynamically indexed register array
+              v = cx[0];
 #pragma unroll
-            for (int c = 0; c < 6; c++)
-              xs[q][c] = cached(jj[h]) ? cx[c] : __builtin_bit_cast(T, xb[h][c]);
+              for (int cc = 1; cc < 6; cc++)
+                if (jc[k] == cc) v = cx[cc];
+            }
+            xs[jq[k]][jc[k]] = v;
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
