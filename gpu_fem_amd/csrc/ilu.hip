@@ -946,7 +946,8 @@ constexpr int kSweepThreads = 256;        // 4 wavefronts per workgroup
 // 2 -> 3.64, 1 -> 5.17; with 12 register batches (ILU(16)): 2 -> 3.55,
 // 3 -> 3.26, 4 -> 3.31, 5 -> 3.61; with 64-block passes (ILU(96)): 3 -> 5.99,
 // 4 -> 5.95, 6 -> 6.13, 8 -> 6.19; with the entries of a dependency in
-// consecutive lanes (less memory traffic per wavefront): 3 -> 5.26, 4 -> 5.12
+// consecutive lanes (fewer memory requests per wavefront): 3 -> 5.26,
+// 4 -> 5.12, 5 -> 5.29, 6 -> 5.30
 #ifndef GPUFEM_SWEEP_WAVES_PER_CU
 #define GPUFEM_SWEEP_WAVES_PER_CU 4
 #endif
