@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "symbolic.h"
@@ -122,66 +123,85 @@ int pattern_permute(int n, const int* rowp, const int* cols, const int* perm,
   return GF_OK;
 }
 
+namespace {
+struct int2s {
+  int row, lev;
+};
+}  // namespace
+
 int iluk_symbolic(int n, const int* rowp, const int* cols, int lev_fill,
                   std::vector<int>& frowp, std::vector<int>& fcols,
                   std::vector<int>* flevs) {
-  // Row-by-row level-of-fill factorisation with a sorted linked list over
-  // column indices for the active row (next[] == n terminates the list).
+  // Row-by-row level-of-fill factorisation.  The active row's levels live in
+  // a dense array (a row stamp per column marks it present in row i), its
+  // pivots (columns left of the diagonal, existing or created by fill) come
+  // off a min-heap in ascending order, and the columns right of the
+  // diagonal are collected unsorted and sorted once when the row is
+  // emitted.  A pivot k merges the U part of row k with
+  //   lev(i, c) = min(lev(i, c), lev(i, k) + lev(k, c) + 1),
+  // keeping a new column only if that level is at most lev_fill; every
+  // pivot below c is applied before c itself is taken as a pivot, so its
+  // level is final by then.  The work is one constant-time update per
+  // (row, pivot, U entry of the pivot row), with no sorted-list walk.
   frowp.assign(n + 1, 0);
   fcols.clear();
   std::vector<int> levs, udiag(n);
   fcols.reserve((size_t)rowp[n] * 2);
   levs.reserve((size_t)rowp[n] * 2);
-  std::vector<int> next(n + 1, -1), lev(n, 0);
-  const int HEAD = n;  // sentinel head index
+  // per column: {row stamp, level}, side by side so an update touches one
+  // cache line
+  std::vector<int2s> st(n, int2s{-1, 0});
+  std::vector<int> heap, ucols;
+  auto heap_push = [&](int c) {
+    heap.push_back(c);
+    std::push_heap(heap.begin(), heap.end(), std::greater<int>());
+  };
   for (int i = 0; i < n; i++) {
-    // seed the list with row i of A plus the diagonal
-    int prev = HEAD;
-    bool has_diag = false;
-    int a0 = rowp[i], a1 = rowp[i + 1];
-    for (int k = a0; k < a1; k++) {
-      int c = cols[k];
-      if (c == i) has_diag = true;
-      if (!has_diag && c > i) {
-        next[prev] = i;
-        lev[i] = 0;
-        prev = i;
-        has_diag = true;
-      }
-      next[prev] = c;
-      lev[c] = 0;
-      prev = c;
+    heap.clear();
+    ucols.clear();
+    st[i] = int2s{i, 0};
+    for (int k = rowp[i]; k < rowp[i + 1]; k++) {
+      const int c = cols[k];
+      if (c == i) continue;
+      st[c] = int2s{i, 0};
+      if (c < i)
+        heap.push_back(c);
+      else
+        ucols.push_back(c);
     }
-    if (!has_diag) {
-      next[prev] = i;
-      lev[i] = 0;
-      prev = i;
-    }
-    next[prev] = n;  // end
-    // eliminate with each pivot k < i in ascending order
-    int k = next[HEAD];
-    while (k < i) {
-      int lk = lev[k];
-      int pos = k;  // insertion cursor: list is sorted, new cols are > k
-      for (int kk = udiag[k] + 1; kk < frowp[k + 1]; kk++) {
-        int c = fcols[kk];
-        int nl = lk + levs[kk] + 1;
-        while (next[pos] < c) pos = next[pos];
-        if (next[pos] == c) {
-          if (lev[c] > nl) lev[c] = nl;
+    std::make_heap(heap.begin(), heap.end(), std::greater<int>());
+    while (!heap.empty()) {
+      std::pop_heap(heap.begin(), heap.end(), std::greater<int>());
+      const int k = heap.back();
+      heap.pop_back();
+      const int lk = st[k].lev;
+      fcols.push_back(k);
+      levs.push_back(lk);
+      const int* uc = fcols.data() + udiag[k] + 1;
+      const int* ul = levs.data() + udiag[k] + 1;
+      const int nu = frowp[k + 1] - (udiag[k] + 1);
+      for (int t = 0; t < nu; t++) {
+        const int c = uc[t];
+        const int nl = lk + ul[t] + 1;
+        int2s& e = st[c];
+        if (e.row == i) {
+          if (e.lev > nl) e.lev = nl;
         } else if (nl <= lev_fill) {
-          next[c] = next[pos];
-          next[pos] = c;
-          lev[c] = nl;
+          e = int2s{i, nl};
+          if (c < i)
+            heap_push(c);
+          else
+            ucols.push_back(c);
         }
       }
-      k = next[k];
     }
-    // emit the row
-    for (int c = next[HEAD]; c < n; c = next[c]) {
-      if (c == i) udiag[i] = (int)fcols.size();
+    udiag[i] = (int)fcols.size();
+    fcols.push_back(i);
+    levs.push_back(0);
+    std::sort(ucols.begin(), ucols.end());
+    for (int c : ucols) {
       fcols.push_back(c);
-      levs.push_back(lev[c]);
+      levs.push_back(st[c].lev);
     }
     frowp[i + 1] = (int)fcols.size();
   }
