@@ -12,7 +12,7 @@ Fixtures (all numeric data, no reference source text):
       include/linalg/reordering_utils.h and include/linalg/bsr_utils.h by
       oracle/build_ref.sh, on small seeded meshes (plates and a cylinder)
   iluk_highfill_ref.npz  the reference's computeILUk at the levels of fill
-      the solver runs (k = 6, 12, 16, 32, 56) on AMD- and RCM-ordered
+      the solver runs (k = 6, 12, 16, 32, 56, 64, 96) on AMD- and RCM-ordered
       patterns of mid-size meshes: input pattern, output row pointers and
       level counts per row, and the SHA-256 of the output column and level
       arrays (int32, little-endian)
@@ -72,7 +72,7 @@ def symbolic():
     np.savez_compressed(os.path.join(HERE, "symbolic_ref.npz"), **data)
 
 
-HIGHFILL_K = (6, 12, 16, 32, 56)
+HIGHFILL_K = (6, 12, 16, 32, 56, 64, 96)
 
 
 def highfill_meshes():

@@ -251,11 +251,12 @@ def _sha(a):
 
 
 @pytest.mark.parametrize("name", HIGH_NAMES)
-@pytest.mark.parametrize("k", [6, 12, 16, 32, 56])
+@pytest.mark.parametrize("k", [6, 12, 16, 32, 56, 64, 96])
 def test_iluk_high_fill_against_reference_golden(name, k):
-    # the levels of fill the solver runs (ILU(32) serving, ILU(56) on the
-    # 400 x 400 plate), against the reference's own computeILUk
-    # (include/linalg/bsr_utils.h:678-807) compiled from its source
+    # the levels of fill the solver runs (ILU(64) serving, ILU(96) on the
+    # 400 x 400 plate, and the lower fills of earlier settings), against the
+    # reference's own computeILUk (include/linalg/bsr_utils.h:678-807)
+    # compiled from its source
     g = lambda key: HIGH[f"{name}/{key}"]
     prp, pcp = g("rowp"), g("cols")
     frp, fcp = G.iluk_symbolic(prp, pcp, k)
@@ -265,6 +266,6 @@ def test_iluk_high_fill_against_reference_golden(name, k):
     np.testing.assert_array_equal(orp, frp)
     np.testing.assert_array_equal(ocp, fcp)
     assert _sha(olv) == str(g(f"ilu{k}_sha_levs"))
-    if name == "plate100_amd" and k == 56:
-        # the fixture exercises an incomplete pattern at k = 56
+    if name == "plate100_amd" and k in (56, 96):
+        # the fixture exercises incomplete patterns at k = 56 and 96
         assert frp[-1] < int(g("full_nnzb")[0])
