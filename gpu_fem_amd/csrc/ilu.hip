@@ -1026,13 +1026,16 @@ __global__ void fill_sentinel_kernel(size_t n, T* x) {
 // running wavefront, or earlier in the same chunk, so the sweep cannot
 // deadlock whatever the number of resident workgroups.
 //
-// Per row: the dependencies' entries (up to kGather blocks per pass) are
-// loaded together and parked in LDS, so a row costs about two memory round
-// trips (column indices, then dependency entries) whatever its length; then
-// slot t = lane / 6 (0..9) takes blocks k0+t, k0+t+10, ... with r = lane % 6
-// the scalar row; the 6x6 diagonal block is solved by lanes 0-5 with
-// shuffles.  The last kCache rows solved by the wavefront are kept in
-// registers (the rows of a chain depend on the rows just before them).
+// Per row, in passes of up to kGather dependency blocks: the pass's factor
+// blocks are loaded into registers, then its dependencies' entries (six
+// consecutive lanes per dependency) are loaded together, polled until none
+// holds the sentinel and parked in LDS, so a pass costs about two memory
+// round trips (column indices, then dependency entries) whatever its
+// length; slot t = lane / 6 (0..9) takes blocks t, t+10, ... of the pass
+// with r = lane % 6 the scalar row; the 6x6 diagonal block is solved by
+// lanes 0-5 (v_readlane broadcasts with one row per wavefront).  The row
+// last solved by the wavefront is kept in registers (a row often depends on
+// the row just before it).
 //
 // fill_next (lower sweep only, may be null): the entries of the row in the
 // upper sweep's output are set to the sentinel once the row's rhs is read,
