@@ -2,8 +2,9 @@
 // pattern permutation, ILU(k) fill, element->block map, transpose map and
 // the level schedules used by the device ILU and triangular sweeps.
 //
-// These are the product's own implementations (array-based, O(nnz) or
-// O(fill) work); they must reproduce the reference's patterns bit for bit:
+// These are the product's own implementations (array-based; O(nnz) work,
+// and for ILU(k) one step per (row, pivot, U entry of the pivot row)); they
+// must reproduce the reference's patterns bit for bit:
 //   gf_node_graph       bsr_utils.h:419-476
 //   gf_rcm_order        reordering_utils.h:189-291
 //   gf_pattern_permute  gmres.cu:242-252 (BSRMatApplyPerm)
